@@ -1,0 +1,180 @@
+/*
+ * wust_hip.h -- C ABI of the MI355X (gfx950) Helmholtz solve-and-gradient engine.
+ *
+ * One opaque handle (wust_helmholtz) owns one device-resident operator: the
+ * assembled nine-point PML stencil of one sound-speed model at one frequency
+ * and its block-tridiagonal factorization.  Every entry point takes plain
+ * pointers and sizes; device pointers are HIP device memory, host pointers
+ * are marked "host".  `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  All entry points return a wust_status; on failure
+ * wust_last_error() returns a thread-local message and wust_status_id()
+ * maps the status to the reference's error identifier ("WUST:InvalidInput",
+ * "WUST:SingularSchurBlock", ...; Runtime/solver/block_lu_mex_utils.h:37-159).
+ *
+ * Array layouts follow the reference (MATLAB column-major):
+ *   grid arrays  [Ny, Nx]       element (y, x)    at  y + Ny*x
+ *   fields       [Ny, Nx, R]    element (y, x, r) at  y + Ny*x + Ny*Nx*r
+ *   observations [Ntx, Nrx]     element (t, q)    at  t + Ntx*q
+ * complex values are interleaved (re, im) pairs of double.
+ * Node indices (tx_index / rx_index) are MATLAB 1-based linear indices
+ * y + Ny*x + 1 (Runtime/matlab/wust_geometry.m:27).
+ */
+#ifndef WUST_HIP_H
+#define WUST_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WUST_ABI_VERSION 1
+
+typedef enum {
+    WUST_OK = 0,
+    WUST_INVALID_INPUT = 1,        /* WUST:InvalidInput */
+    WUST_INVALID_OUTPUT = 2,       /* WUST:InvalidOutput */
+    WUST_HIP_ERROR = 3,            /* WUST:CUDAError analogue (HIP runtime) */
+    WUST_LIBRARY_ERROR = 4,        /* WUST:CUBLASError / WUST:CUSOLVERError analogue */
+    WUST_SINGULAR_SCHUR_BLOCK = 5, /* WUST:SingularSchurBlock */
+    WUST_NUMERICAL_FAILURE = 6,    /* WUST:NumericalFailure */
+    WUST_RESOURCE_CLEANUP = 7,     /* WUST:ResourceCleanup */
+    WUST_RUNTIME_ERROR = 8,        /* WUST:RuntimeError */
+    WUST_NOT_CONVERGED = 9         /* iterative refinement missed its tolerance */
+} wust_status;
+
+/* Arithmetic of the factor and solves. */
+typedef enum {
+    /* Hand-written gfx950 kernels: complex-FP32 MFMA block factor and sweeps,
+     * FP64 matrix-free residual, FP64 refinement accumulation. */
+    WUST_MODE_REFINED = 0,
+    /* The same complex-FP32 kernels without refinement. */
+    WUST_MODE_C64 = 1,
+    /* The reference's GPU algorithm on vendor libraries, unrefined: one
+     * rocSOLVER getrf+getrs per Schur block and one rocBLAS cgemm plus a
+     * tridiagonal kernel per grid column (the "plain complex-FP32 block LU"
+     * of Runtime/solver/decompBlockLU.cu / applyBlockLU.cu). */
+    WUST_MODE_PLAIN = 2
+} wust_mode;
+
+typedef struct wust_helmholtz wust_helmholtz;
+
+/* Grid, frequency and PML description: the scalar arguments of
+ * HelmholtzSolver(x, y, vel, atten, f, signConvention, a0, L_PML, options)
+ * (Runtime/solver/HelmholtzSolver.m:32).  Node coordinates are
+ * x_i = x0 + i*dx, y_i = y0 + i*dy. */
+typedef struct {
+    int32_t ny, nx;
+    double x0, dx, y0, dy;
+    double frequency_hz;
+    double sign_convention;     /* wust_oracle passes -1 */
+    double pml_strength;        /* a0 */
+    double pml_m;               /* L_PML [m] */
+    double stencil_vmin, stencil_vmax;   /* options.stencil_bounds */
+    int32_t dispersion;         /* 0 continuum, 1 kwave-ldr9 */
+    double ldr9_time_step_s;    /* options.dispersion.time_step_s */
+    double ldr9_reference_speed_mps;
+} wust_grid_desc;
+
+typedef struct {
+    int32_t ny, nx, n, m, npad;    /* n = ny-2 block size, m = nx-2 blocks, npad = padded n */
+    int32_t mode;
+    double b, d, e;                /* stencil parameters */
+    double assembly_ms, factor_ms; /* device time, hipEvent-timed */
+    double min_pivot_ratio;        /* smallest |pivot| / max |column entry| seen in the factor */
+    int64_t factor_bytes;          /* device bytes held by the factors */
+} wust_info;
+
+typedef struct {
+    int32_t iterations;            /* refinement corrections applied (0 = one plain solve) */
+    double initial_residual;       /* max over RHS of ||b - H u0|| / ||b|| after the first solve */
+    double final_residual;         /* same after the last correction */
+    double solve_ms;               /* device time of the whole call */
+} wust_solve_stats;
+
+/* ---- library ------------------------------------------------------------------ */
+int wust_abi_version(void);
+const char *wust_status_id(int status);
+const char *wust_last_error(void);
+
+/* Optimal nine-point mass coefficients, b = 5/6 fixed (host computation).
+ * Replaces stencilOptParams(vmin, vmax, f, h, g) (Runtime/solver/stencilOptParams.m:1-38). */
+int wust_stencil_opt_params(double vmin, double vmax, double f, double h, double g,
+                            double *b, double *d, double *e);
+
+/* ---- operator ------------------------------------------------------------------- */
+/* Assemble the operator on the device from the sound speed (and optional
+ * attenuation, may be NULL = 0), then factor it.  d_vel, d_atten: device
+ * double [Ny, Nx].  Replaces the HelmholtzSolver constructor
+ * (Runtime/solver/HelmholtzSolver.m:32-172), ninePointCoefficients
+ * (Runtime/solver/ninePointCoefficients.m:1-74) and decompBlockLU
+ * (Runtime/solver/decompBlockLU.cu:173-396).  The factor is held by the
+ * handle (the reference returns it as invT). */
+int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const double *d_atten,
+                          int mode, void *stream, wust_helmholtz **out);
+int wust_helmholtz_destroy(wust_helmholtz *h);
+int wust_helmholtz_info(const wust_helmholtz *h, wust_info *info);
+
+/* The nine coefficient arrays of ninePointCoefficients in the reference
+ * shapes, written as complex128 device arrays (any pointer may be NULL):
+ * Dd [Ny,Nx]; Dl, Du [Ny-1,Nx]; Ld, Ud [Ny,Nx-1]; Ll, Lu, Ul, Uu [Ny-1,Nx-1]. */
+int wust_helmholtz_coefficients(const wust_helmholtz *h, void *Dd, void *Dl, void *Du, void *Ld,
+                                void *Ll, void *Lu, void *Ud, void *Ul, void *Uu, void *stream);
+
+/* Solve H u = src (adjoint = 0) or H^H u = src (adjoint = 1) for R right-hand
+ * sides.  d_src, d_out: device complex128 [Ny, Nx, R]; sources must vanish
+ * on the outer Dirichlet ring, and the output is zero there (the reference
+ * block LU keeps those degrees of freedom at zero,
+ * Runtime/solver/applyBlockLU.cu:7-10).  In WUST_MODE_REFINED the FP64
+ * residual is driven below `tol` (relative, max over RHS) within `max_iter`
+ * corrections.  Replaces HelmholtzSolver.solve
+ * (Runtime/solver/HelmholtzSolver.m:174-203) and applyBlockLU
+ * (Runtime/solver/applyBlockLU.cu:291-481).  stats may be NULL. */
+int wust_helmholtz_solve(wust_helmholtz *h, const void *d_src, void *d_out, int64_t R, int adjoint,
+                         double tol, int max_iter, void *stream, wust_solve_stats *stats);
+
+/* v = (C .* q') .* u, the second output of solve (HelmholtzSolver.m:197-199).
+ * d_u, d_v: device complex128 [Ny, Nx, R]. */
+int wust_helmholtz_virtual_sources(const wust_helmholtz *h, const void *d_u, void *d_v, int64_t R,
+                                   void *stream);
+
+/* Exact mass-stencil derivative M v (adjoint = 0, output border zeroed) or
+ * M^T v (adjoint = 1, input border zeroed first).  Replaces massApply
+ * (Runtime/solver/HelmholtzSolver.m:205-220).  Device complex128 [Ny,Nx,R]. */
+int wust_helmholtz_mass_apply(const wust_helmholtz *h, const void *d_in, void *d_out, int64_t R,
+                              int adjoint, void *stream);
+
+/* ---- objective / derivatives ------------------------------------------------------ */
+/* Loss, gradient and (retain != 0) curvature state of wust_oracle
+ * (Runtime/matlab/wust_oracle.m:1-86) for the handle's model and frequency.
+ *   tx_index, rx_index: host int64, 1-based node indices (n_tx, n_rx entries)
+ *   d_Y:    device complex128 [n_tx, n_rx] observations of this frequency
+ *   d_mask: device uint8 [n_tx, n_rx] (nonzero = valid channel)
+ *   batch:  source batch size (cfg.source_batch_size); <= 0 means all
+ *   d_grad: device double [Ny, Nx], written when need_gradient != 0
+ *   loss:   host double
+ * Source scale, residual and loss are FP64; refinement as in solve(). */
+int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, const int64_t *rx_index,
+                     int64_t n_rx, const void *d_Y, const uint8_t *d_mask, int64_t batch,
+                     int need_gradient, int retain, double tol, int max_iter, double *loss,
+                     void *d_grad, void *stream);
+
+/* Fitted source scales a* (host complex128, n_tx entries) of the last
+ * wust_oracle_eval call that retained state. */
+int wust_oracle_scales(const wust_helmholtz *h, void *scales_host);
+
+/* kappa = sum_b || -(H^{-1} M (v_b .* direction))[rx] a_b .* mask ||^2 with
+ * the retained v and frozen a (Runtime/matlab/directional_curvature.m:1-26).
+ * d_direction: device double [Ny, Nx]; d_mask: device uint8 [n_tx, n_rx]
+ * (the receiver mask of this frequency); kappa: host double. */
+int wust_directional_curvature(wust_helmholtz *h, const double *d_direction, const uint8_t *d_mask,
+                               double tol, int max_iter, double *kappa, void *stream);
+
+/* Drop the retained forward fields (the reference clears its cache after the
+ * last curvature consumer, Runtime/matlab/wust_reconstruct.m:61). */
+int wust_oracle_release(wust_helmholtz *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WUST_HIP_H */

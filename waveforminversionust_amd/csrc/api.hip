@@ -1,0 +1,686 @@
+// C ABI of the engine (declared in include/wust_hip.h): handle lifetime,
+// validation, refinement driver, oracle (loss/gradient) and curvature drivers.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "fields.h"
+#include "plain.h"
+
+using namespace wust;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string &msg) {
+    g_last_error = msg;
+    return status;
+}
+
+template <typename F>
+int guarded(F &&fn) {
+    try {
+        fn();
+        g_last_error.clear();
+        return WUST_OK;
+    } catch (const Error &e) {
+        return fail(e.status, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(WUST_RUNTIME_ERROR, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(WUST_RUNTIME_ERROR, e.what());
+    }
+}
+
+template <typename T>
+T *dalloc(int64_t count) {
+    void *p = nullptr;
+    if (count <= 0) return nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(T) * (size_t)count);
+    if (e != hipSuccess)
+        throw Error(WUST_HIP_ERROR, std::string("hipMalloc(") + std::to_string(sizeof(T) * count) +
+                                         " bytes): " + hipGetErrorString(e));
+    return static_cast<T *>(p);
+}
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// Householder least squares for the 2-column stencil fit
+// (Runtime/solver/stencilOptParams.m:33-37 solves [S2 S3] [d; e] = S4 - b S1 with MATLAB's QR "\").
+void stencil_params(double vmin, double vmax, double f, double h, double g, double &b, double &d, double &e) {
+    const int na = 100, nw = 10;
+    const double pi = M_PI;
+    double Gmin = vmin / (f * h), Gmax = vmax / (f * h);
+    std::vector<double> c0, c1, rhs;
+    b = 5.0 / 6.0;
+    for (int m = 1; m <= na; ++m)       // column-major flatten of [nw, na] meshgrid
+        for (int n = 1; n <= nw; ++n) {
+            double th = (m - 1) * pi / (4.0 * (na - 1));
+            double G = 1.0 / (1.0 / Gmax + ((double)(n - 1) / (nw - 1)) * (1.0 / Gmin - 1.0 / Gmax));
+            double P = std::cos(g * 2 * pi * std::cos(th) / G);
+            double Q = std::cos(2 * pi * std::sin(th) / G);
+            double S1 = (1 + 1 / (g * g)) * (G * G) * (1 - P - Q + P * Q);
+            double S2 = (pi * pi) * (2 - P - Q);
+            double S3 = (2 * pi * pi) * (1 - P * Q);
+            double S4 = 2 * pi * pi + (G * G) * ((1 + 1 / (g * g)) * P * Q - P - Q / (g * g));
+            c0.push_back(S2);
+            c1.push_back(S3);
+            rhs.push_back(S4 - b * S1);
+        }
+    const size_t M = rhs.size();
+    std::vector<double> *cols[2] = {&c0, &c1};
+    double R[2][2] = {{0, 0}, {0, 0}};
+    for (int k = 0; k < 2; ++k) {
+        std::vector<double> &a = *cols[k];
+        double norm = 0;
+        for (size_t i = k; i < M; ++i) norm += a[i] * a[i];
+        norm = std::sqrt(norm);
+        double alpha = a[k] > 0 ? -norm : norm;
+        std::vector<double> v(M, 0.0);
+        for (size_t i = k; i < M; ++i) v[i] = a[i];
+        v[k] -= alpha;
+        double vv = 0;
+        for (size_t i = k; i < M; ++i) vv += v[i] * v[i];
+        auto reflect = [&](std::vector<double> &x) {
+            double s = 0;
+            for (size_t i = k; i < M; ++i) s += v[i] * x[i];
+            s = 2 * s / vv;
+            for (size_t i = k; i < M; ++i) x[i] -= s * v[i];
+        };
+        for (int j = k; j < 2; ++j) reflect(*cols[j]);
+        reflect(rhs);
+        for (int j = k; j < 2; ++j) R[k][j] = (*cols[j])[k];
+    }
+    double x1 = rhs[1] / R[1][1];
+    double x0 = (rhs[0] - R[0][1] * x1) / R[0][0];
+    d = x0;
+    e = x1;
+}
+
+struct Retained {
+    cd *u = nullptr;       // forward field (solve layout c128)
+    cd *scale = nullptr;   // device [Rpad]
+    int first = 0, nb = 0, Rpad = 0;
+};
+
+}  // namespace
+
+struct wust_helmholtz {
+    Operator op;
+    PlainLib plain;
+    int ws_Rpad = 0;
+    cd *b64 = nullptr, *t64 = nullptr, *u64 = nullptr, *lam64 = nullptr;
+    cf *s32 = nullptr, *d32 = nullptr;
+    cf *p_src = nullptr, *p_u = nullptr, *p_tmp = nullptr;
+    double *norms = nullptr;    // [2][Rpad]
+    double *loss_r = nullptr;   // [Rpad]
+    cd *scale = nullptr;        // [Rpad]
+    int *idx = nullptr;         // scratch for node coordinates: 4 * max(ntx, nrx)
+    int idx_cap = 0;
+    std::vector<Retained> retained;
+    std::vector<int> rx_x, rx_y;
+    int ret_ntx = 0, ret_nrx = 0;
+    std::vector<cd> scales_host;
+
+    ~wust_helmholtz() {
+        release_retained();
+        release_ws();
+        dfree(op.ck2);
+        dfree(op.vsm);
+        dfree(op.w64);
+        dfree(op.w32);
+        dfree(op.pages);
+        dfree(op.work);
+        dfree(op.status);
+        dfree(idx);
+    }
+    void release_retained() {
+        for (auto &r : retained) {
+            dfree(r.u);
+            dfree(r.scale);
+        }
+        retained.clear();
+    }
+    void release_ws() {
+        dfree(b64);
+        dfree(t64);
+        dfree(u64);
+        dfree(lam64);
+        dfree(s32);
+        dfree(d32);
+        dfree(p_src);
+        dfree(p_u);
+        dfree(p_tmp);
+        dfree(norms);
+        dfree(loss_r);
+        dfree(scale);
+        ws_Rpad = 0;
+    }
+    int64_t field_elems(int Rpad) const { return (int64_t)op.m * op.npad * Rpad; }
+    void ensure_ws(int Rpad) {
+        if (Rpad <= ws_Rpad) return;
+        release_ws();
+        int64_t cnt = field_elems(Rpad);
+        b64 = dalloc<cd>(cnt);
+        t64 = dalloc<cd>(cnt);
+        u64 = dalloc<cd>(cnt);
+        lam64 = dalloc<cd>(cnt);
+        if (op.mode == WUST_MODE_PLAIN) {
+            p_src = dalloc<cf>(cnt);
+            p_u = dalloc<cf>(cnt);
+            p_tmp = dalloc<cf>((int64_t)op.npad * Rpad);
+        } else {
+            s32 = dalloc<cf>(cnt);
+            d32 = dalloc<cf>(cnt);
+        }
+        norms = dalloc<double>(2 * (int64_t)Rpad);
+        loss_r = dalloc<double>(Rpad);
+        scale = dalloc<cd>(Rpad);
+        ws_Rpad = Rpad;
+    }
+    void ensure_idx(int count) {
+        if (count <= idx_cap) return;
+        dfree(idx);
+        idx = dalloc<int>(4 * (int64_t)count);
+        idx_cap = count;
+    }
+    uint32_t read_status(hipStream_t s) {
+        uint32_t st = 0;
+        WUST_HIP_CHECK(hipMemcpyAsync(&st, op.status, sizeof(st), hipMemcpyDeviceToHost, s));
+        WUST_HIP_CHECK(hipStreamSynchronize(s));
+        return st;
+    }
+    void clear_status(hipStream_t s) { WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, sizeof(uint32_t), s)); }
+
+    // Solve in the solve layout: b (c128, Rpad) -> out (c128).  b is not modified.
+    wust_solve_stats solve_internal(const cd *b, cd *out, int R, int Rpad, bool adjoint, double tol, int max_iter,
+                                    bool want_residual, hipStream_t s) {
+        wust_solve_stats st{};
+        const int64_t cnt = field_elems(Rpad);
+        double *nb2 = norms, *nr2 = norms + Rpad;
+        auto residual_pass = [&](const cd *u_old, const cf *d, cd *u_new, cf *res32) {
+            WUST_HIP_CHECK(hipMemsetAsync(nr2, 0, sizeof(double) * Rpad, s));
+            ResidualArgs a{};
+            a.w64 = op.w64;
+            a.u_old = u_old;
+            a.d32 = d;
+            a.b64 = b;
+            a.u_new = u_new;
+            a.res32 = res32;
+            a.norm2 = nr2;
+            a.n = op.n;
+            a.m = op.m;
+            a.npad = op.npad;
+            a.Rpad = Rpad;
+            a.adjoint = adjoint ? 1 : 0;
+            launch_residual(a, s);
+            std::vector<double> h(2 * (size_t)Rpad);
+            WUST_HIP_CHECK(hipMemcpyAsync(h.data(), norms, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+            WUST_HIP_CHECK(hipStreamSynchronize(s));
+            double worst = 0;
+            for (int r = 0; r < R; ++r)
+                if (h[r] > 0) worst = std::max(worst, std::sqrt(h[Rpad + r] / h[r]));
+                else if (h[Rpad + r] > 0) worst = INFINITY;
+            return worst;
+        };
+        // right-hand-side norms
+        {
+            WUST_HIP_CHECK(hipMemsetAsync(nb2, 0, sizeof(double) * Rpad, s));
+            ResidualArgs a{};
+            a.w64 = op.w64;
+            a.b64 = b;
+            a.norm2 = nb2;
+            a.n = op.n;
+            a.m = op.m;
+            a.npad = op.npad;
+            a.Rpad = Rpad;
+            launch_residual(a, s);
+        }
+        if (op.mode == WUST_MODE_PLAIN) {
+            solve64_to_plain32(b, p_src, op.m, op.npad, R, Rpad, s);
+            solve_plain(op, plain, p_src, p_u, p_tmp, R, adjoint, s);
+            plain32_to_solve64(p_u, out, op.m, op.npad, R, Rpad, s);
+            if (want_residual) {
+                st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr);
+            }
+            return st;
+        }
+        demote(b, s32, cnt, s);
+        solve_engine(op, s32, d32, Rpad, adjoint, s);
+        if (op.mode == WUST_MODE_C64) {
+            promote(d32, out, cnt, s);
+            if (want_residual) st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr);
+            return st;
+        }
+        // refinement: U_new = U_old + D; R = b - H U_new; D = solve(R)
+        cd *bufs[2] = {out, t64 == out ? u64 : t64};
+        const cd *u_old = nullptr;
+        int cur = 0;
+        int it = 0;
+        for (;; ++it) {
+            double rel = residual_pass(u_old, d32, bufs[cur], s32);
+            if (it == 0) st.initial_residual = rel;
+            st.final_residual = rel;
+            if (!(rel > tol) || it >= max_iter) break;
+            solve_engine(op, s32, d32, Rpad, adjoint, s);
+            u_old = bufs[cur];
+            cur ^= 1;
+        }
+        st.iterations = it;
+        if (bufs[cur] != out)
+            WUST_HIP_CHECK(hipMemcpyAsync(out, bufs[cur], sizeof(cd) * cnt, hipMemcpyDeviceToDevice, s));
+        if (!std::isfinite(st.final_residual))
+            throw Error(WUST_NUMERICAL_FAILURE, "non-finite residual in the refined solve");
+        if (st.final_residual > tol && max_iter > 0)
+            throw Error(WUST_NOT_CONVERGED, "refinement stopped at relative residual " +
+                                                std::to_string(st.final_residual) + " > tol " + std::to_string(tol));
+        return st;
+    }
+
+    void node_coords(const int64_t *index, int64_t count, std::vector<int> &xs, std::vector<int> &ys, bool allow_ring,
+                     const char *what) {
+        xs.resize(count);
+        ys.resize(count);
+        const int64_t N = (int64_t)op.ny * op.nx;
+        for (int64_t i = 0; i < count; ++i) {
+            int64_t node = index[i] - 1;
+            require(node >= 0 && node < N, std::string(what) + " index outside the grid");
+            int y = (int)(node % op.ny), x = (int)(node / op.ny);
+            bool ring = (y == 0 || y == op.ny - 1 || x == 0 || x == op.nx - 1);
+            if (ring) {
+                require(allow_ring, std::string(what) + " must sample interior grid points");
+                xs[i] = -1;
+                ys[i] = -1;
+            } else {
+                xs[i] = x - 1;
+                ys[i] = y - 1;
+            }
+        }
+    }
+};
+
+extern "C" {
+
+int wust_abi_version(void) { return WUST_ABI_VERSION; }
+
+const char *wust_status_id(int status) {
+    switch (status) {
+        case WUST_OK: return "WUST:OK";
+        case WUST_INVALID_INPUT: return "WUST:InvalidInput";
+        case WUST_INVALID_OUTPUT: return "WUST:InvalidOutput";
+        case WUST_HIP_ERROR: return "WUST:HIPError";
+        case WUST_LIBRARY_ERROR: return "WUST:LibraryError";
+        case WUST_SINGULAR_SCHUR_BLOCK: return "WUST:SingularSchurBlock";
+        case WUST_NUMERICAL_FAILURE: return "WUST:NumericalFailure";
+        case WUST_RESOURCE_CLEANUP: return "WUST:ResourceCleanup";
+        case WUST_RUNTIME_ERROR: return "WUST:RuntimeError";
+        case WUST_NOT_CONVERGED: return "WUST:NotConverged";
+        default: return "WUST:Unknown";
+    }
+}
+
+const char *wust_last_error(void) { return g_last_error.c_str(); }
+
+int wust_stencil_opt_params(double vmin, double vmax, double f, double h, double g, double *b, double *d, double *e) {
+    return guarded([&] {
+        require(b && d && e, "null output pointer");
+        require(std::isfinite(vmin) && std::isfinite(vmax) && vmin > 0 && vmax > vmin,
+                "stencil_bounds must be a positive ordered [min,max] pair");
+        require(f > 0 && h > 0 && g > 0, "frequency and spacings must be positive");
+        stencil_params(vmin, vmax, f, h, g, *b, *d, *e);
+    });
+}
+
+int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const double *d_atten, int mode,
+                          void *stream, wust_helmholtz **out) {
+    if (out) *out = nullptr;
+    return guarded([&] {
+        require(desc && d_vel && out, "null argument");
+        require(mode == WUST_MODE_REFINED || mode == WUST_MODE_C64 || mode == WUST_MODE_PLAIN, "unknown mode");
+        const wust_grid_desc &g = *desc;
+        require(g.ny >= 3 && g.nx >= 3, "Expected a positive finite [ny,nx] medium (min size 3)");
+        require(std::isfinite(g.dx) && std::isfinite(g.dy) && g.dx > 0 && g.dy > 0, "Uniform increasing grids required");
+        require(std::isfinite(g.frequency_hz) && g.frequency_hz > 0, "frequency must be positive");
+        require(std::isfinite(g.pml_m) && g.pml_m > 0, "pml_m must be positive");
+        require(std::isfinite(g.pml_strength), "pml_strength must be finite");
+        require(std::isfinite(g.stencil_vmin) && std::isfinite(g.stencil_vmax) && g.stencil_vmin > 0 &&
+                    g.stencil_vmax > g.stencil_vmin,
+                "stencil_bounds must be a positive ordered [min,max] pair");
+        if (g.dispersion) {
+            require(d_atten == nullptr, "LDR9 requires zero attenuation");
+            require(std::fabs(g.dy / g.dx - 1) < 1e-10, "LDR9 requires square grid cells");
+            require(g.ldr9_time_step_s > 0 && g.ldr9_reference_speed_mps > 0, "LDR9 needs time step and reference speed");
+        }
+        hipStream_t s = (hipStream_t)stream;
+        std::unique_ptr<wust_helmholtz> h(new wust_helmholtz());
+        Operator &op = h->op;
+        op.ny = g.ny;
+        op.nx = g.nx;
+        op.n = g.ny - 2;
+        op.m = g.nx - 2;
+        op.npad = (int)round_up(op.n, 32);
+        op.p = op.m / 2;
+        op.mode = mode;
+        stencil_params(g.stencil_vmin, g.stencil_vmax, g.frequency_hz, g.dx, g.dy / g.dx, op.b, op.d, op.e);
+        AssemblyParams &ap = op.ap;
+        ap.ny = g.ny;
+        ap.nx = g.nx;
+        ap.npad = op.npad;
+        ap.x0 = g.x0;
+        ap.dx = g.dx;
+        ap.y0 = g.y0;
+        ap.dy = g.dy;
+        ap.f = g.frequency_hz;
+        ap.sgn = (g.sign_convention > 0) ? 1.0 : (g.sign_convention < 0 ? -1.0 : 0.0);
+        ap.a0 = g.pml_strength;
+        ap.L = g.pml_m;
+        ap.b = op.b;
+        ap.d = op.d;
+        ap.e = op.e;
+        ap.dispersion = g.dispersion;
+        ap.ldr9_dt = g.ldr9_time_step_s;
+        ap.ldr9_cr = g.ldr9_reference_speed_mps;
+        const int64_t nodes = (int64_t)g.ny * g.nx;
+        op.ck2 = dalloc<cd>(nodes);
+        op.vsm = dalloc<cd>(nodes);
+        op.w64 = dalloc<cd>(9 * (int64_t)op.m * op.n);
+        op.w32 = dalloc<cf>(9 * (int64_t)op.m * op.npad);
+        op.status = dalloc<uint32_t>(4);
+        const int64_t pg = (int64_t)op.npad * op.npad;
+        op.pages = dalloc<cf>(pg * op.m);
+        op.work = dalloc<cf>(4 * pg);
+        op.factor_bytes = (int64_t)sizeof(cf) * pg * op.m;
+        WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, 4 * sizeof(uint32_t), s));
+
+        hipEvent_t e0, e1, e2;
+        WUST_HIP_CHECK(hipEventCreate(&e0));
+        WUST_HIP_CHECK(hipEventCreate(&e1));
+        WUST_HIP_CHECK(hipEventCreate(&e2));
+        WUST_HIP_CHECK(hipEventRecord(e0, s));
+        launch_assembly(ap, d_vel, d_atten, op.ck2, op.vsm, op.w64, op.w32, op.status, s);
+        WUST_HIP_CHECK(hipEventRecord(e1, s));
+        uint32_t st = h->read_status(s);
+        if (st & ST_NONFINITE) throw Error(WUST_NUMERICAL_FAILURE, "LDR-9 produced invalid wavenumber or derivative");
+        if (mode == WUST_MODE_PLAIN) factor_plain(op, h->plain, s);
+        else factor_engine(op, s);
+        WUST_HIP_CHECK(hipEventRecord(e2, s));
+        WUST_HIP_CHECK(hipEventSynchronize(e2));
+        float ms1 = 0, ms2 = 0;
+        WUST_HIP_CHECK(hipEventElapsedTime(&ms1, e0, e1));
+        WUST_HIP_CHECK(hipEventElapsedTime(&ms2, e1, e2));
+        op.assembly_ms = ms1;
+        op.factor_ms = ms2;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+        st = h->read_status(s);
+        if (st & (ST_ZERO_PIVOT | ST_NONFINITE_PIVOT))
+            throw Error(WUST_SINGULAR_SCHUR_BLOCK, "zero or non-finite pivot in a Schur block");
+        *out = h.release();
+    });
+}
+
+int wust_helmholtz_destroy(wust_helmholtz *h) {
+    return guarded([&] { delete h; });
+}
+
+int wust_helmholtz_info(const wust_helmholtz *h, wust_info *info) {
+    return guarded([&] {
+        require(h && info, "null argument");
+        const Operator &op = h->op;
+        info->ny = op.ny;
+        info->nx = op.nx;
+        info->n = op.n;
+        info->m = op.m;
+        info->npad = op.npad;
+        info->mode = op.mode;
+        info->b = op.b;
+        info->d = op.d;
+        info->e = op.e;
+        info->assembly_ms = op.assembly_ms;
+        info->factor_ms = op.factor_ms;
+        info->min_pivot_ratio = op.min_pivot_ratio;
+        info->factor_bytes = op.factor_bytes;
+    });
+}
+
+int wust_helmholtz_coefficients(const wust_helmholtz *h, void *Dd, void *Dl, void *Du, void *Ld, void *Ll, void *Lu,
+                                void *Ud, void *Ul, void *Uu, void *stream) {
+    return guarded([&] {
+        require(h != nullptr, "null handle");
+        launch_export(h->op.ny, h->op.nx, h->op.w64, (cd *)Dd, (cd *)Dl, (cd *)Du, (cd *)Ld, (cd *)Ll, (cd *)Lu,
+                      (cd *)Ud, (cd *)Ul, (cd *)Uu, (hipStream_t)stream);
+    });
+}
+
+int wust_helmholtz_solve(wust_helmholtz *h, const void *d_src, void *d_out, int64_t R, int adjoint, double tol,
+                         int max_iter, void *stream, wust_solve_stats *stats) {
+    return guarded([&] {
+        require(h && d_src && d_out, "null argument");
+        require(R >= 1 && R <= (1 << 20), "number of right-hand sides out of range");
+        require(d_src != d_out, "solve output must not alias its source");
+        require(max_iter >= 0, "max_iter must be non-negative");
+        hipStream_t s = (hipStream_t)stream;
+        hipEvent_t e0, e1;
+        WUST_HIP_CHECK(hipEventCreate(&e0));
+        WUST_HIP_CHECK(hipEventCreate(&e1));
+        WUST_HIP_CHECK(hipEventRecord(e0, s));
+        const int Rpad = (int)round_up(R, 16);
+        h->ensure_ws(Rpad);
+        ext_to_solve64((const cd *)d_src, h->b64, h->op.ny, h->op.nx, h->op.npad, (int)R, Rpad, nullptr, s);
+        wust_solve_stats st =
+            h->solve_internal(h->b64, h->u64, (int)R, Rpad, adjoint != 0, tol, max_iter, stats != nullptr, s);
+        solve64_to_ext(h->u64, (cd *)d_out, h->op.ny, h->op.nx, h->op.npad, (int)R, Rpad, s);
+        WUST_HIP_CHECK(hipEventRecord(e1, s));
+        WUST_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        WUST_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        st.solve_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (stats) *stats = st;
+    });
+}
+
+int wust_helmholtz_virtual_sources(const wust_helmholtz *h, const void *d_u, void *d_v, int64_t R, void *stream) {
+    return guarded([&] {
+        require(h && d_u && d_v, "null argument");
+        require(R >= 1, "R must be positive");
+        launch_virtual_sources((const cd *)d_u, h->op.vsm, (cd *)d_v, (int64_t)h->op.ny * h->op.nx, R,
+                               (hipStream_t)stream);
+    });
+}
+
+int wust_helmholtz_mass_apply(const wust_helmholtz *h, const void *d_in, void *d_out, int64_t R, int adjoint,
+                              void *stream) {
+    return guarded([&] {
+        require(h && d_in && d_out, "null argument");
+        require(d_in != d_out, "mass_apply output must not alias its input");
+        require(R >= 1, "R must be positive");
+        launch_mass_apply((const cd *)d_in, (cd *)d_out, h->op.ny, h->op.nx, R, h->op.d, h->op.e, adjoint ? 1 : 0,
+                          (hipStream_t)stream);
+    });
+}
+
+int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, const int64_t *rx_index, int64_t n_rx,
+                     const void *d_Y, const uint8_t *d_mask, int64_t batch, int need_gradient, int retain, double tol,
+                     int max_iter, double *loss, void *d_grad, void *stream) {
+    return guarded([&] {
+        require(h && tx_index && rx_index && d_Y && d_mask && loss, "null argument");
+        require(n_tx >= 1 && n_rx >= 1, "empty transmitter or receiver set");
+        require(!need_gradient || d_grad, "gradient output required");
+        hipStream_t s = (hipStream_t)stream;
+        Operator &op = h->op;
+        std::vector<int> tx_x, tx_y;
+        h->node_coords(tx_index, n_tx, tx_x, tx_y, false, "tx_index");
+        h->node_coords(rx_index, n_rx, h->rx_x, h->rx_y, true, "rx_index");
+        if (batch <= 0 || batch > n_tx) batch = n_tx;
+        h->release_retained();
+        h->scales_host.assign((size_t)n_tx, mkd(0, 0));
+        h->ret_ntx = (int)n_tx;
+        h->ret_nrx = (int)n_rx;
+        const int cap = (int)std::max(n_tx, n_rx);
+        h->ensure_idx(cap);
+        int *rxx = h->idx, *rxy = h->idx + cap, *txx = h->idx + 2 * cap, *txy = h->idx + 3 * cap;
+        WUST_HIP_CHECK(hipMemcpyAsync(rxx, h->rx_x.data(), sizeof(int) * n_rx, hipMemcpyHostToDevice, s));
+        WUST_HIP_CHECK(hipMemcpyAsync(rxy, h->rx_y.data(), sizeof(int) * n_rx, hipMemcpyHostToDevice, s));
+        WUST_HIP_CHECK(hipMemcpyAsync(txx, tx_x.data(), sizeof(int) * n_tx, hipMemcpyHostToDevice, s));
+        WUST_HIP_CHECK(hipMemcpyAsync(txy, tx_y.data(), sizeof(int) * n_tx, hipMemcpyHostToDevice, s));
+        if (need_gradient)
+            WUST_HIP_CHECK(hipMemsetAsync(d_grad, 0, sizeof(double) * (size_t)op.ny * op.nx, s));
+        double total = 0;
+        for (int64_t first = 0; first < n_tx; first += batch) {
+            const int nb = (int)std::min<int64_t>(batch, n_tx - first);
+            const int Rpad = (int)round_up(nb, 16);
+            h->ensure_ws(Rpad);
+            const int64_t cnt = h->field_elems(Rpad);
+            // unit impulses at the batch's transmitters (Runtime/matlab/wust_oracle.m:31-34)
+            WUST_HIP_CHECK(hipMemsetAsync(h->b64, 0, sizeof(cd) * cnt, s));
+            launch_impulses(h->b64, txx + first, txy + first, nb, op.npad, Rpad, s);
+            cd *ufwd = h->u64;
+            if (retain) ufwd = dalloc<cd>(cnt);
+            try {
+                h->solve_internal(h->b64, ufwd, nb, Rpad, false, tol, max_iter, false, s);
+                // source fit, residual, loss and the adjoint source
+                h->clear_status(s);
+                cd *adj = h->b64;  // forward RHS no longer needed
+                WUST_HIP_CHECK(hipMemsetAsync(adj, 0, sizeof(cd) * cnt, s));
+                SourceFitArgs fa{};
+                fa.u = ufwd;
+                fa.Y = (const cd *)d_Y;
+                fa.mask = d_mask;
+                fa.rx_x = rxx;
+                fa.rx_y = rxy;
+                fa.scale = h->scale;
+                fa.loss = h->loss_r;
+                fa.adj_src = need_gradient ? adj : nullptr;
+                fa.status = op.status;
+                fa.ntx = (int)n_tx;
+                fa.nrx = (int)n_rx;
+                fa.tx_offset = (int)first;
+                fa.npad = op.npad;
+                fa.Rpad = Rpad;
+                launch_source_fit(fa, nb, s);
+                std::vector<double> lr(nb);
+                std::vector<cd> sc(nb);
+                WUST_HIP_CHECK(hipMemcpyAsync(lr.data(), h->loss_r, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+                WUST_HIP_CHECK(hipMemcpyAsync(sc.data(), h->scale, sizeof(cd) * nb, hipMemcpyDeviceToHost, s));
+                uint32_t st = h->read_status(s);
+                if (st & ST_SCALE_FAILURE)
+                    throw Error(WUST_NUMERICAL_FAILURE, "Unusable source-scale denominator/numerator");
+                for (int r = 0; r < nb; ++r) {
+                    total += lr[r];
+                    h->scales_host[first + r] = sc[r];
+                }
+                if (need_gradient) {
+                    h->solve_internal(adj, h->lam64, nb, Rpad, true, tol, max_iter, false, s);
+                    GradientArgs ga{};
+                    ga.u = ufwd;
+                    ga.lam = h->lam64;
+                    ga.vsm = op.vsm;
+                    ga.scale = h->scale;
+                    ga.grad = (double *)d_grad;
+                    ga.d = op.d;
+                    ga.e = op.e;
+                    ga.ny = op.ny;
+                    ga.n = op.n;
+                    ga.m = op.m;
+                    ga.npad = op.npad;
+                    ga.Rpad = Rpad;
+                    ga.R = nb;
+                    launch_gradient(ga, s);
+                }
+                if (retain) {
+                    Retained rt;
+                    rt.u = ufwd;
+                    rt.scale = dalloc<cd>(Rpad);
+                    WUST_HIP_CHECK(hipMemsetAsync(rt.scale, 0, sizeof(cd) * Rpad, s));
+                    WUST_HIP_CHECK(hipMemcpyAsync(rt.scale, h->scale, sizeof(cd) * nb, hipMemcpyDeviceToDevice, s));
+                    rt.first = (int)first;
+                    rt.nb = nb;
+                    rt.Rpad = Rpad;
+                    h->retained.push_back(rt);
+                    ufwd = nullptr;
+                }
+            } catch (...) {
+                if (retain && ufwd) dfree(ufwd);
+                throw;
+            }
+        }
+        WUST_HIP_CHECK(hipStreamSynchronize(s));
+        if (!std::isfinite(total)) throw Error(WUST_NUMERICAL_FAILURE, "Nonfinite objective");
+        *loss = total;
+    });
+}
+
+int wust_oracle_scales(const wust_helmholtz *h, void *scales_host) {
+    return guarded([&] {
+        require(h && scales_host, "null argument");
+        std::memcpy(scales_host, h->scales_host.data(), sizeof(cd) * h->scales_host.size());
+    });
+}
+
+int wust_directional_curvature(wust_helmholtz *h, const double *d_direction, const uint8_t *d_mask, double tol,
+                               int max_iter, double *kappa, void *stream) {
+    return guarded([&] {
+        require(h && d_direction && d_mask && kappa, "null argument");
+        require(!h->retained.empty(), "no retained oracle state (call wust_oracle_eval with retain)");
+        hipStream_t s = (hipStream_t)stream;
+        Operator &op = h->op;
+        const int cap = h->idx_cap;
+        int *rxx = h->idx, *rxy = h->idx + cap;
+        double total = 0;
+        for (auto &rt : h->retained) {
+            h->ensure_ws(rt.Rpad);
+            CurvSrcArgs ca{};
+            ca.u = rt.u;
+            ca.vsm = op.vsm;
+            ca.direction = d_direction;
+            ca.src = h->b64;
+            ca.d = op.d;
+            ca.e = op.e;
+            ca.ny = op.ny;
+            ca.n = op.n;
+            ca.m = op.m;
+            ca.npad = op.npad;
+            ca.Rpad = rt.Rpad;
+            launch_curvature_source(ca, s);
+            h->solve_internal(h->b64, h->lam64, rt.nb, rt.Rpad, false, tol, max_iter, false, s);
+            CurvGatherArgs ga{};
+            ga.u = h->lam64;
+            ga.scale = rt.scale;
+            ga.mask = d_mask;
+            ga.rx_x = rxx;
+            ga.rx_y = rxy;
+            ga.out = h->loss_r;
+            ga.ntx = h->ret_ntx;
+            ga.nrx = h->ret_nrx;
+            ga.tx_offset = rt.first;
+            ga.npad = op.npad;
+            ga.Rpad = rt.Rpad;
+            launch_curvature_gather(ga, rt.nb, s);
+            std::vector<double> part(rt.nb);
+            WUST_HIP_CHECK(hipMemcpyAsync(part.data(), h->loss_r, sizeof(double) * rt.nb, hipMemcpyDeviceToHost, s));
+            WUST_HIP_CHECK(hipStreamSynchronize(s));
+            for (double v : part) total += v;
+        }
+        *kappa = total;
+    });
+}
+
+int wust_oracle_release(wust_helmholtz *h) {
+    return guarded([&] {
+        require(h != nullptr, "null handle");
+        h->release_retained();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,42 @@
+// Device state of one Helmholtz operator and the host drivers that act on it.
+#pragma once
+#include <vector>
+
+#include "assembly.h"
+#include "wust_common.h"
+
+namespace wust {
+
+// Device layouts (all column-major within a block):
+//   w64   [9][m][n]        FP64 per-node stencil rows of the interior nodes
+//   w32   [9][m][npad]     complex-FP32 block coefficients (ring couplings and padding zeroed)
+//   pages [m][npad][npad]  inverse Schur blocks X_j (twisted: left chain j < p, right chain
+//                          j > p, middle j = p); padding rows/cols hold the identity
+//   solve fields  [m][npad][Rpad]   interior block x, row y, RHS r fastest
+struct Operator {
+    int ny = 0, nx = 0, n = 0, m = 0, npad = 0, p = 0;  // p: twisted meeting block
+    int mode = 0;
+    double b = 0, d = 0, e = 0;
+    AssemblyParams ap{};
+    cd *ck2 = nullptr, *vsm = nullptr, *w64 = nullptr;
+    cf *w32 = nullptr;
+    cf *pages = nullptr;
+    cf *work = nullptr;           // factor ping-pong buffers: [2 chains][2][npad*npad]
+    uint32_t *status = nullptr;   // device status words: [0] flags, [2..3] pivot ratios (float)
+    double assembly_ms = 0, factor_ms = 0, min_pivot_ratio = 1.0;
+    int64_t factor_bytes = 0;
+
+    const cf *coef(int dir, int block) const { return w32 + ((int64_t)dir * m + block) * npad; }
+    cf *page(int j) const { return pages + (int64_t)j * npad * npad; }
+};
+
+// ---- factor (twisted block-tridiagonal Schur recurrence + Gauss-Jordan inverses) ----
+void factor_engine(Operator &op, hipStream_t s);
+// single-chain Schur block T = D_j - L_j X_{j-1} U_{j-1} (T = D_0 for j = 0), X = op.page
+void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s);
+
+// ---- complex-FP32 block solve: out = H^{-1} src (or H^{-H} src), solve layout ----
+// src, out: [m][npad][Rpad] complex-FP32; out must not alias src.
+void solve_engine(const Operator &op, const cf *src, cf *out, int Rpad, bool adjoint, hipStream_t s);
+
+}  // namespace wust

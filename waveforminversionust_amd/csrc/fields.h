@@ -1,0 +1,75 @@
+#pragma once
+#include "wust_common.h"
+
+namespace wust {
+
+__host__ __device__ __forceinline__ cd tod_any(cd v) { return v; }
+__host__ __device__ __forceinline__ cd tod_any(cf v) { return tod(v); }
+
+struct ResidualArgs {
+    const cd *w64;        // [9][m][n]
+    const cd *u_old;      // solve layout c128 or null
+    const cf *d32;        // correction c64 or null
+    const cd *b64;        // right-hand side c128 or null (= 0)
+    cd *u_new;            // U_old + D (may be null)
+    cf *res32;            // B - H U_new as c64 (may be null)
+    cd *res64;            // same in c128 (may be null)
+    double *norm2;        // per-RHS sum |res|^2 accumulated with atomics (may be null)
+    int n, m, npad, Rpad, adjoint;
+};
+
+struct SourceFitArgs {
+    const cd *u;          // forward field, solve layout
+    const cd *Y;          // [ntx, nrx] column-major
+    const uint8_t *mask;  // [ntx, nrx]
+    const int *rx_x, *rx_y;  // interior coordinates, -1 for receivers on the ring
+    cd *scale;            // [R]
+    double *loss;         // [R]
+    cd *adj_src;          // solve layout c128, pre-zeroed (or null)
+    uint32_t *status;
+    int ntx, nrx, tx_offset, npad, Rpad;
+};
+
+struct GradientArgs {
+    const cd *u, *lam;    // solve layout c128
+    const cd *vsm;        // [Ny*Nx]
+    const cd *scale;      // [R]
+    double *grad;         // [Ny*Nx], accumulated
+    double d, e;
+    int ny, n, m, npad, Rpad, R;
+};
+
+struct CurvSrcArgs {
+    const cd *u;          // retained forward field, solve layout
+    const cd *vsm;
+    const double *direction;  // [Ny*Nx]
+    cd *src;              // solve layout c128
+    double d, e;
+    int ny, n, m, npad, Rpad;
+};
+
+struct CurvGatherArgs {
+    const cd *u;
+    const cd *scale;
+    const uint8_t *mask;
+    const int *rx_x, *rx_y;
+    double *out;          // [R]
+    int ntx, nrx, tx_offset, npad, Rpad;
+};
+
+void ext_to_solve64(const cd *ext, cd *out, int ny, int nx, int npad, int R, int Rpad, double *norm2, hipStream_t s);
+void solve64_to_ext(const cd *in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s);
+void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s);
+void launch_residual(const ResidualArgs &a, hipStream_t s);
+void promote(const cf *in, cd *out, int64_t count, hipStream_t s);
+void demote(const cd *in, cf *out, int64_t count, hipStream_t s);
+void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s);
+void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s);
+void launch_gradient(const GradientArgs &a, hipStream_t s);
+void launch_curvature_source(const CurvSrcArgs &a, hipStream_t s);
+void launch_curvature_gather(const CurvGatherArgs &a, int R, hipStream_t s);
+void launch_mass_apply(const cd *in, cd *out, int ny, int nx, int64_t R, double d, double e, int adjoint,
+                       hipStream_t s);
+void launch_virtual_sources(const cd *u, const cd *vsm, cd *v, int64_t nodes, int64_t R, hipStream_t s);
+
+}  // namespace wust

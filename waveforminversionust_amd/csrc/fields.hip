@@ -1,0 +1,395 @@
+// FP64 field kernels: layout conversion, matrix-free nine-point residual with
+// the refinement update fused in, source fit and loss, slowness gradient, mass
+// stencil, virtual sources and the curvature source.  All of these are
+// HBM-bound; each thread owns one (node, RHS) value and RHS is the fastest
+// index of every solve-layout array so that the stencil neighbours, the
+// coefficient rows and the stores are coalesced.
+#include "fields.h"
+
+namespace wust {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// external [R][Nx][Ny] complex128  <->  solve layout [m][npad][Rpad]
+// ---------------------------------------------------------------------------
+template <typename TO>
+__device__ __forceinline__ TO conv_out(cd v);
+template <>
+__device__ __forceinline__ cd conv_out<cd>(cd v) { return v; }
+template <>
+__device__ __forceinline__ cf conv_out<cf>(cd v) { return tof(v); }
+
+template <typename TO>
+__global__ void __launch_bounds__(256) ext_to_solve_kernel(const cd *__restrict__ ext, TO *__restrict__ out, int ny,
+                                                           int nx, int n, int npad, int R, int Rpad,
+                                                           double *__restrict__ norm2) {
+    __shared__ cd tile[32][33];
+    const int xi = blockIdx.z;
+    const int y0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    // load: lanes along y (contiguous in ext)
+    for (int k = ty; k < 32; k += 8) {
+        int yi = y0 + tx, r = r0 + k;
+        cd v = mkd(0, 0);
+        if (yi < n && r < R) v = ext[(int64_t)r * nx * ny + (int64_t)(xi + 1) * ny + (yi + 1)];
+        tile[k][tx] = v;
+    }
+    __syncthreads();
+    // store: lanes along r (contiguous in the solve layout)
+    double acc = 0;
+    for (int k = ty; k < 32; k += 8) {
+        int yi = y0 + k, r = r0 + tx;
+        if (yi < npad && r < Rpad) {
+            cd v = tile[tx][k];
+            out[((int64_t)xi * npad + yi) * Rpad + r] = conv_out<TO>(v);
+            acc += abs2(v);
+        }
+    }
+    if (norm2) {
+        __shared__ double part[8][33];
+        part[ty][tx] = acc;
+        __syncthreads();
+        if (ty == 0) {
+            double s = 0;
+            for (int k = 0; k < 8; ++k) s += part[k][tx];
+            int r = r0 + tx;
+            if (r < Rpad && s != 0) atomicAdd(norm2 + r, s);
+        }
+    }
+}
+
+template <typename TI>
+__global__ void __launch_bounds__(256) solve_to_ext_kernel(const TI *__restrict__ in, cd *__restrict__ ext, int ny,
+                                                           int nx, int n, int npad, int R, int Rpad) {
+    __shared__ cd tile[32][33];
+    const int x = blockIdx.z;  // full grid column 0..nx-1
+    const int y0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const bool xin = (x >= 1 && x <= nx - 2);
+    for (int k = ty; k < 32; k += 8) {
+        int y = y0 + k, r = r0 + tx;
+        cd v = mkd(0, 0);
+        if (xin && y >= 1 && y <= ny - 2 && r < R) v = tod_any(in[((int64_t)(x - 1) * npad + (y - 1)) * Rpad + r]);
+        tile[k][tx] = v;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        int y = y0 + tx, r = r0 + k;
+        if (y < ny && r < R) ext[(int64_t)r * nx * ny + (int64_t)x * ny + y] = tile[tx][k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused refinement update + FP64 residual:  U_new = U_old + D;  Rs = B - H U_new
+// (or H^H).  Couplings leaving the interior see zero (Dirichlet ring).
+// ---------------------------------------------------------------------------
+__constant__ int c_dy[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
+__constant__ int c_dx[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
+__constant__ int c_opp[9] = {DC, DS, DN, DE, DSE, DNE, DW, DSW, DNW};
+
+__global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
+    const int r = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int yi = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int xi = blockIdx.z;
+    const int n = a.n, m = a.m, npad = a.npad, Rpad = a.Rpad;
+    double contrib = 0;
+    if (yi < npad && r < Rpad) {
+        const int64_t o = ((int64_t)xi * npad + yi) * Rpad + r;
+        if (yi >= n) {
+            if (a.u_new) a.u_new[o] = mkd(0, 0);
+            if (a.res32) a.res32[o] = mk(0.f, 0.f);
+            if (a.res64) a.res64[o] = mkd(0, 0);
+        } else {
+            const int64_t plane = (int64_t)m * n;
+            cd hu = mkd(0, 0);
+            cd self = mkd(0, 0);
+#pragma unroll
+            for (int dd = 0; dd < 9; ++dd) {
+                int yy = yi + c_dy[dd], xx = xi + c_dx[dd];
+                if (yy < 0 || yy >= n || xx < 0 || xx >= m) continue;
+                int64_t on = ((int64_t)xx * npad + yy) * Rpad + r;
+                cd u = mkd(0, 0);
+                if (a.u_old) u = a.u_old[on];
+                if (a.d32) u = u + tod(a.d32[on]);
+                if (dd == DC) self = u;
+                cd w;
+                if (!a.adjoint) w = a.w64[dd * plane + (int64_t)xi * n + yi];
+                else w = conjd(a.w64[c_opp[dd] * plane + (int64_t)xx * n + yy]);
+                hu = hu + w * u;
+            }
+            cd b = a.b64 ? a.b64[o] : mkd(0, 0);
+            cd res = b - hu;
+            if (a.u_new) a.u_new[o] = self;
+            if (a.res32) a.res32[o] = tof(res);
+            if (a.res64) a.res64[o] = res;
+            contrib = abs2(res);
+        }
+    }
+    if (a.norm2) {
+        __shared__ double part[16][17];
+        part[threadIdx.x >> 4][threadIdx.x & 15] = contrib;
+        __syncthreads();
+        if (threadIdx.x < 16) {
+            double s = 0;
+            for (int k = 0; k < 16; ++k) s += part[k][threadIdx.x];
+            int rr = blockIdx.x * 16 + threadIdx.x;
+            if (rr < Rpad && s != 0) atomicAdd(a.norm2 + rr, s);
+        }
+    }
+}
+
+__global__ void promote_kernel(const cf *__restrict__ in, cd *__restrict__ out, int64_t count) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < count) out[i] = tod(in[i]);
+}
+__global__ void demote_kernel(const cd *__restrict__ in, cf *__restrict__ out, int64_t count) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < count) out[i] = tof(in[i]);
+}
+
+// unit impulses: B[xi][yi][r] = 1 at the source node of RHS r (B pre-zeroed)
+__global__ void impulse_kernel(cd *__restrict__ b, const int *__restrict__ sx, const int *__restrict__ sy, int R,
+                               int npad, int Rpad) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    b[((int64_t)sx[r] * npad + sy[r]) * Rpad + r] = mkd(1, 0);
+}
+
+// ---------------------------------------------------------------------------
+// source fit (Runtime/matlab/wust_oracle.m:43-64): one block per RHS
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double s = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
+    return s;
+}
+
+__global__ void __launch_bounds__(256) source_fit_kernel(SourceFitArgs a) {
+    const int r = blockIdx.x;
+    const int t = a.tx_offset + r;
+    __shared__ double sh[8];
+    double nr = 0, ni = 0, den = 0;
+    for (int q = threadIdx.x; q < a.nrx; q += blockDim.x) {
+        double mk_ = a.mask[t + (int64_t)a.ntx * q] ? 1.0 : 0.0;
+        cd p = mkd(0, 0);
+        if (a.rx_x[q] >= 0) p = a.u[((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r];
+        p = mk_ * p;
+        cd d = mk_ * a.Y[t + (int64_t)a.ntx * q];
+        cd cp = conjd(p) * d;
+        nr += cp.re;
+        ni += cp.im;
+        den += abs2(p);
+    }
+    nr = block_sum(nr, sh);
+    ni = block_sum(ni, sh);
+    den = block_sum(den, sh);
+    cd num = mkd(nr, ni);
+    bool ok = isfinite(den) && den > 2.2250738585072014e-308 && isfinite(nr) && isfinite(ni);
+    cd scale = ok ? cdiv(num, mkd(den, 0)) : mkd(0, 0);
+    if (!(isfinite(scale.re) && isfinite(scale.im))) ok = false;
+    if (!ok) {
+        if (threadIdx.x == 0) atomicOr(a.status, (uint32_t)ST_SCALE_FAILURE);
+        scale = mkd(0, 0);
+    }
+    double loss = 0;
+    for (int q = threadIdx.x; q < a.nrx; q += blockDim.x) {
+        double mk_ = a.mask[t + (int64_t)a.ntx * q] ? 1.0 : 0.0;
+        cd p = mkd(0, 0);
+        if (a.rx_x[q] >= 0) p = a.u[((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r];
+        p = mk_ * p;
+        cd d = mk_ * a.Y[t + (int64_t)a.ntx * q];
+        cd res = mk_ * (p * scale - d);
+        loss += abs2(res);
+        if (a.adj_src && a.rx_x[q] >= 0) a.adj_src[((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r] = res;
+    }
+    loss = block_sum(loss, sh);
+    if (threadIdx.x == 0) {
+        a.scale[r] = scale;
+        a.loss[r] = 0.5 * loss;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// gradient (Runtime/matlab/wust_oracle.m:66-68):
+// g(y,x) += sum_r -Re( conj(a_r vsm u) .* (K lambda) ),  K = mass stencil,
+// lambda = 0 on the Dirichlet ring (massApply(., true) zeroes the input border)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) gradient_kernel(GradientArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int yi = blockIdx.x * 4 + wave;
+    const int xi = blockIdx.y;
+    if (yi >= a.n) return;
+    const int npad = a.npad, Rpad = a.Rpad;
+    const double kc = 1 - a.d - a.e, ke = a.d / 4, kd = a.e / 4;
+    const cd v = a.vsm[(int64_t)(xi + 1) * a.ny + (yi + 1)];
+    double acc = 0;
+    for (int r = lane; r < a.R; r += 64) {
+        auto L = [&](int yy, int xx) -> cd {
+            if (yy < 0 || yy >= a.n || xx < 0 || xx >= a.m) return mkd(0, 0);
+            return a.lam[((int64_t)xx * npad + yy) * Rpad + r];
+        };
+        cd kl = kc * L(yi, xi);
+        kl = kl + ke * (L(yi - 1, xi) + L(yi + 1, xi) + L(yi, xi - 1) + L(yi, xi + 1));
+        kl = kl + kd * (L(yi - 1, xi - 1) + L(yi + 1, xi - 1) + L(yi - 1, xi + 1) + L(yi + 1, xi + 1));
+        cd u = a.u[((int64_t)xi * npad + yi) * Rpad + r];
+        cd av = a.scale[r] * (v * u);
+        // -Re(conj(av) * kl)
+        acc -= av.re * kl.re + av.im * kl.im;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) a.grad[(int64_t)(xi + 1) * a.ny + (yi + 1)] += acc;
+}
+
+// curvature source: src(i, r) = sum_j K(j - i) vsm(j) u(j, r) delta(j), j interior
+__global__ void __launch_bounds__(256) curvature_source_kernel(CurvSrcArgs a) {
+    const int r = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int yi = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int xi = blockIdx.z;
+    if (yi >= a.npad || r >= a.Rpad) return;
+    const int64_t o = ((int64_t)xi * a.npad + yi) * a.Rpad + r;
+    if (yi >= a.n) {
+        a.src[o] = mkd(0, 0);
+        return;
+    }
+    const double kc = 1 - a.d - a.e, ke = a.d / 4, kd = a.e / 4;
+    cd s = mkd(0, 0);
+    for (int dd = 0; dd < 9; ++dd) {
+        int yy = yi + c_dy[dd], xx = xi + c_dx[dd];
+        if (yy < 0 || yy >= a.n || xx < 0 || xx >= a.m) continue;
+        double kw = (dd == DC) ? kc : ((dd == DN || dd == DS || dd == DW || dd == DE) ? ke : kd);
+        int64_t node = (int64_t)(xx + 1) * a.ny + (yy + 1);
+        cd v = a.vsm[node] * a.u[((int64_t)xx * a.npad + yy) * a.Rpad + r];
+        s = s + (kw * a.direction[node]) * v;
+    }
+    a.src[o] = s;
+}
+
+// kappa partials: sum_q | u(rx_q, r) a_r mask |^2  (one block per RHS)
+__global__ void __launch_bounds__(256) curvature_gather_kernel(CurvGatherArgs a) {
+    const int r = blockIdx.x;
+    const int t = a.tx_offset + r;
+    __shared__ double sh[8];
+    double s = 0;
+    for (int q = threadIdx.x; q < a.nrx; q += blockDim.x) {
+        if (!a.mask[t + (int64_t)a.ntx * q] || a.rx_x[q] < 0) continue;
+        cd u = a.u[((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r];
+        s += abs2(u * a.scale[r]);
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) a.out[r] = s;
+}
+
+// mass stencil on external-layout fields (Runtime/solver/HelmholtzSolver.m:205-220)
+__global__ void mass_apply_kernel(const cd *__restrict__ in, cd *__restrict__ out, int ny, int nx, int64_t R,
+                                  double d, double e, int adjoint) {
+    int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)ny * nx * R;
+    if (idx >= total) return;
+    int y = (int)(idx % ny);
+    int64_t t = idx / ny;
+    int x = (int)(t % nx);
+    int64_t r = t / nx;
+    bool border = (y == 0 || y == ny - 1 || x == 0 || x == nx - 1);
+    if (!adjoint && border) {
+        out[idx] = mkd(0, 0);
+        return;
+    }
+    const double kc = 1 - d - e, ke = d / 4, kd = e / 4;
+    cd s = mkd(0, 0);
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            int yy = y + dy, xx = x + dx;
+            if (yy < 0 || yy >= ny || xx < 0 || xx >= nx) continue;
+            if (adjoint && (yy == 0 || yy == ny - 1 || xx == 0 || xx == nx - 1)) continue;
+            double kw = (dy == 0 && dx == 0) ? kc : ((dy == 0 || dx == 0) ? ke : kd);
+            s = s + kw * in[r * ny * nx + (int64_t)xx * ny + yy];
+        }
+    out[idx] = s;
+}
+
+__global__ void virtual_sources_kernel(const cd *__restrict__ u, const cd *__restrict__ vsm, cd *__restrict__ v,
+                                       int64_t nodes, int64_t R) {
+    int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (idx >= nodes * R) return;
+    v[idx] = vsm[idx % nodes] * u[idx];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void ext_to_solve64(const cd *ext, cd *out, int ny, int nx, int npad, int R, int Rpad, double *norm2, hipStream_t s) {
+    int n = ny - 2, m = nx - 2;
+    dim3 grid((npad + 31) / 32, (Rpad + 31) / 32, m);
+    hipLaunchKernelGGL(ext_to_solve_kernel<cd>, grid, dim3(256), 0, s, ext, out, ny, nx, n, npad, R, Rpad, norm2);
+    WUST_LAUNCH_CHECK();
+}
+void solve64_to_ext(const cd *in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s) {
+    int n = ny - 2;
+    dim3 grid((ny + 31) / 32, (R + 31) / 32, nx);
+    hipLaunchKernelGGL(solve_to_ext_kernel<cd>, grid, dim3(256), 0, s, in, ext, ny, nx, n, npad, R, Rpad);
+    WUST_LAUNCH_CHECK();
+}
+void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s) {
+    int n = ny - 2;
+    dim3 grid((ny + 31) / 32, (R + 31) / 32, nx);
+    hipLaunchKernelGGL(solve_to_ext_kernel<cf>, grid, dim3(256), 0, s, in, ext, ny, nx, n, npad, R, Rpad);
+    WUST_LAUNCH_CHECK();
+}
+void launch_residual(const ResidualArgs &a, hipStream_t s) {
+    dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);
+    hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, s, a);
+    WUST_LAUNCH_CHECK();
+}
+void promote(const cf *in, cd *out, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(promote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
+    WUST_LAUNCH_CHECK();
+}
+void demote(const cd *in, cf *out, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(demote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
+    WUST_LAUNCH_CHECK();
+}
+void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s) {
+    hipLaunchKernelGGL(impulse_kernel, dim3((R + 255) / 256), dim3(256), 0, s, b, sx, sy, R, npad, Rpad);
+    WUST_LAUNCH_CHECK();
+}
+void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s) {
+    hipLaunchKernelGGL(source_fit_kernel, dim3(R), dim3(256), 0, s, a);
+    WUST_LAUNCH_CHECK();
+}
+void launch_gradient(const GradientArgs &a, hipStream_t s) {
+    dim3 grid((a.n + 3) / 4, a.m);
+    hipLaunchKernelGGL(gradient_kernel, grid, dim3(256), 0, s, a);
+    WUST_LAUNCH_CHECK();
+}
+void launch_curvature_source(const CurvSrcArgs &a, hipStream_t s) {
+    dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);
+    hipLaunchKernelGGL(curvature_source_kernel, grid, dim3(256), 0, s, a);
+    WUST_LAUNCH_CHECK();
+}
+void launch_curvature_gather(const CurvGatherArgs &a, int R, hipStream_t s) {
+    hipLaunchKernelGGL(curvature_gather_kernel, dim3(R), dim3(256), 0, s, a);
+    WUST_LAUNCH_CHECK();
+}
+void launch_mass_apply(const cd *in, cd *out, int ny, int nx, int64_t R, double d, double e, int adjoint,
+                       hipStream_t s) {
+    int64_t total = (int64_t)ny * nx * R;
+    hipLaunchKernelGGL(mass_apply_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, in, out, ny, nx, R,
+                       d, e, adjoint);
+    WUST_LAUNCH_CHECK();
+}
+void launch_virtual_sources(const cd *u, const cd *vsm, cd *v, int64_t nodes, int64_t R, hipStream_t s) {
+    int64_t total = nodes * R;
+    hipLaunchKernelGGL(virtual_sources_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, u, vsm, v,
+                       nodes, R);
+    WUST_LAUNCH_CHECK();
+}
+
+}  // namespace wust
