@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Gradient evaluations per second of the Helmholtz solve-and-gradient engine.
+
+Workload (the metric case): 512x512 interior + 24-cell PML (560x560 nodes),
+dx = 0.25 mm, f = 0.8 MHz, 256-element ring at r = 56 mm, every element
+transmits and all receive (256 RHS per solve).  Synthetic data: the model is
+1500 m/s + 40 m/s of Gaussian-filtered seeded noise (sigma 3 mm); the
+observations are the engine's refined forward solve on that model; the
+gradient is evaluated at homogeneous 1500 m/s.
+
+One step = one wust_oracle(nargout=2) call: device assembly from c -> block
+factor -> forward solves of all 256 TX -> FP64 source fit and loss -> adjoint
+solves -> FP64 gradient.  Inputs (c, Y, mask) are resident in HBM before the
+timed region.  "refined" = hand-written complex-FP32 MFMA factor and sweeps
+with FP64 residual refinement; "plain" = the reference's GPU algorithm on
+rocSOLVER/rocBLAS (one getrf+getrs per Schur block, one cgemm per column),
+unrefined.  Accuracy is measured against the FP64 sparse-LU oracle on a
+bounded subset of transmitters (the whole grid, 8 TX), which is also the CPU
+baseline sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "grad evals/s at 512²×256 TX: refined c64 vs plain c64; rel err vs FP64 LU"
+MFMA_FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--tol", type=float, default=1e-12, help="refinement stop (relative FP64 residual)")
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 oracle accuracy / CPU baseline leg")
+    ap.add_argument("--cpu-tx", type=int, default=8, help="transmitters in the CPU/accuracy sample")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the transmitter-sharded multi-GPU layout is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from waveforminversionust_amd import _native as N
+    from waveforminversionust_amd import helmholtz as G
+    from waveforminversionust_amd import workloads as W
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    lib = N.load()
+    case = W.metric_case()
+    f = case.frequencies_hz[0]
+    ny, nx, ntx = case.ny, case.nx, case.n_tx
+    log(f"workload {case.name}: {ny}x{nx} nodes, {ntx} TX, f={f/1e6} MHz")
+
+    # ---- synthetic observations (untimed) ----
+    opts = dict(backend="gpu", stencil_bounds=case.cfg["stencil_bounds"], tol=args.tol, device=dev)
+    truth = G.HelmholtzSolver(case.x_m, case.y_m, torch.as_tensor(case.c_true, device=dev), None, f, -1,
+                              case.cfg["pml_strength"], case.cfg["pml_m"], opts)
+    src = torch.zeros((ntx, nx, ny), dtype=torch.complex128, device=dev)
+    tx0 = torch.as_tensor(case.tx_index - 1, device=dev)
+    src.view(ntx, -1)[torch.arange(ntx, device=dev), tx0] = 1
+    u, _ = truth.solve(src.permute(2, 1, 0), False)
+    rx0 = torch.as_tensor(case.rx_index - 1, device=dev)
+    Y = u.permute(2, 1, 0).reshape(ntx, -1)[:, rx0].contiguous()   # [ntx, nrx]
+    truth.close()
+    del u, src
+    obs = dict(x_m=case.x_m, y_m=case.y_m, shape_yx=(ny, nx), n_tx=ntx, tx_index=case.tx_index,
+               rx_index=case.rx_index, frequencies_hz=[f], Y=Y[:, :, None],
+               mask=torch.ones((ntx, case.n_rx, 1), dtype=torch.bool, device=dev))
+    c0 = torch.full((ny, nx), 1500.0, dtype=torch.float64, device=dev)
+    cfg = dict(case.cfg, mode="refined", tol=args.tol, device=dev)
+
+    def step(mode="refined"):
+        return G.wust_oracle(c0, obs, 0, dict(cfg, mode=mode))
+
+    # ---- warm-up, then one fully profiled step for the phase breakdown ----
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.wust_profile_reset()
+    lib.wust_profile_enable((1 << len(N.PROFILE_CLASSES)) - 1)
+    step()
+    torch.cuda.synchronize()
+    phases = {}
+    for ci, name in enumerate(N.PROFILE_CLASSES):
+        cnt, ms = N.C.c_int64(), N.C.c_double()
+        lib.wust_profile_read(ci, N.C.byref(cnt), N.C.byref(ms))
+        if cnt.value:
+            phases[name] = dict(launches=cnt.value, ms=round(ms.value, 3))
+    dominant = max(("sweep_gemm", "gj_panel"), key=lambda k: phases.get(k, {}).get("ms", 0.0))
+    dom_idx = N.PROFILE_CLASSES.index(dominant)
+    lib.wust_profile_reset()
+
+    # ---- timed region: K refined gradient evaluations (dominant kernel event-timed) ----
+    lib.wust_profile_enable(1 << dom_idx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss_ref, g_ref = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    cnt, ms = N.C.c_int64(), N.C.c_double()
+    lib.wust_profile_read(dom_idx, N.C.byref(cnt), N.C.byref(ms))
+    lib.wust_profile_enable(0)
+    value = args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"refined: {ms_per_step:.1f} ms/grad eval, dominant {dominant}: {cnt.value} launches, {ms.value:.1f} ms")
+
+    # roofline of the dominant kernel (algorithmic flops; n = Ny-2 interior rows per block)
+    n, m, R = ny - 2, nx - 2, ntx
+    if dominant == "sweep_gemm":
+        # each launch = up to two chain steps, each an n x n times n x R complex product (8 flops / MAC)
+        # a solve = 2m-1 chain steps in m+1 launches (twisted sweeps)
+        solves = round(cnt.value / (m + 1))
+        total_flops = 8.0 * n * n * R * (2 * m - 1) * solves
+        flops_per_launch = total_flops / max(cnt.value, 1)
+        per_unit = "8*n^2*R flops per chain step (n=558, R=256); 2m-1 chain steps per solve in m+1 launches"
+    else:
+        total_flops = 8.0 * n ** 3 * m * args.steps   # Gauss-Jordan inverse: n^3 complex MACs per block
+        flops_per_launch = total_flops / max(cnt.value, 1)
+        per_unit = "8*n^3 flops per block inverse (n=558), m=558 blocks per factor"
+    avg_launch_s = ms.value / 1e3 / max(cnt.value, 1)
+    achieved = (total_flops / (ms.value / 1e3)) / 1e12 if ms.value > 0 else 0.0
+    roofline = dict(kernel=dominant, bound="mfma", achieved=round(achieved, 2), peak=MFMA_FP32_PEAK_TFLOPS,
+                    unit="TFLOP/s", frac=round(achieved / MFMA_FP32_PEAK_TFLOPS, 4), traffic=None,
+                    launches=cnt.value, avg_launch_us=round(avg_launch_s * 1e6, 2),
+                    algorithmic_flops_per_launch=round(flops_per_launch / 1e9, 4), per_unit=per_unit)
+
+    # ---- plain comparator ----
+    plain = None
+    if not args.no_plain:
+        step("plain")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            loss_plain, g_plain = step("plain")
+        torch.cuda.synchronize()
+        pe = time.perf_counter() - t0
+        plain = dict(value=round(args.plain_steps / pe, 4), ms_per_step=round(1e3 * pe / args.plain_steps, 1),
+                     steps=args.plain_steps,
+                     refined_over_plain=round(value / (args.plain_steps / pe), 2),
+                     grad_rel_diff_vs_refined=float(torch.linalg.norm(g_plain - g_ref) / torch.linalg.norm(g_ref)))
+        log(f"plain: {plain['ms_per_step']} ms/grad eval -> refined/plain = {plain['refined_over_plain']}x")
+
+    # ---- FP64 oracle: accuracy + CPU baseline on a bounded TX sample ----
+    accuracy, cpu = None, None
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+
+        from oracle import helmholtz_fp64 as H
+        sub = np.linspace(0, ntx - 1, args.cpu_tx).round().astype(int)
+        Yh = Y.cpu().numpy()
+        obs_sub = dict(x_m=case.x_m, y_m=case.y_m, shape_yx=(ny, nx), n_tx=len(sub),
+                       tx_index=case.tx_index[sub], rx_index=case.rx_index, frequencies_hz=[f],
+                       Y=Yh[sub][:, :, None], mask=np.ones((len(sub), case.n_rx, 1), bool))
+        c0h = np.full((ny, nx), 1500.0)
+        cfg_sub = dict(case.cfg, source_batch_size=len(sub))
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            H.HelmholtzFP64(case.x_m, case.y_m, c0h, np.zeros_like(c0h), f, -1, case.cfg["pml_strength"],
+                            case.cfg["pml_m"], dict(stencil_bounds=case.cfg["stencil_bounds"]))
+            t_factor = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            lo, go, so = H.wust_oracle(c0h, obs_sub, 0, cfg_sub, 3)
+            t_sample = time.perf_counter() - t0
+        # model: one grad eval = forward factor + adjoint factor + per-TX (forward + adjoint solves, fit, gradient)
+        per_tx = max(t_sample - 2 * t_factor, 0.0) / len(sub)
+        t_full = 2 * t_factor + ntx * per_tx
+        cpu = dict(value=round(1.0 / t_full, 6), unit="grad evals/s", cores=1, kind="port",
+                   sample=(f"FP64 oracle (scipy SuperLU, 1 thread) on the full 560x560 grid with {len(sub)} of "
+                           f"256 TX: factor {t_factor:.2f} s, sample eval {t_sample:.2f} s; "
+                           f"extrapolated 2*factor + 256*(per-TX {per_tx:.3f} s) = {t_full:.1f} s per grad eval"))
+        accuracy = {}
+        for mode in ("refined", "c64", "plain"):
+            obs_dev = dict(obs_sub, Y=torch.as_tensor(obs_sub["Y"], device=dev),
+                           mask=torch.as_tensor(obs_sub["mask"], device=dev))
+            lg, gg = G.wust_oracle(c0, obs_dev, 0, dict(cfg_sub, mode=mode, tol=args.tol, device=dev))
+            gg = gg.cpu().numpy()
+            # forward fields of the sample TX against the oracle's (interior nodes)
+            s = G.HelmholtzSolver(case.x_m, case.y_m, c0, None, f, -1, case.cfg["pml_strength"], case.cfg["pml_m"],
+                                  dict(opts, mode=mode))
+            srcs = np.zeros((ny * nx, len(sub)), np.complex128)
+            srcs[case.tx_index[sub] - 1, np.arange(len(sub))] = 1
+            ug = s.solve(srcs.reshape(ny, nx, -1, order="F"), False)[0].cpu().numpy()
+            s.close()
+            vo = np.concatenate(so.virtual, axis=2)
+            uo = vo / so.solver.virtual_source_multiplier[:, :, None]
+            inner = (slice(1, -1), slice(1, -1))
+            accuracy[mode] = dict(
+                field_rel_err=float(np.linalg.norm(ug[inner] - uo[inner]) / np.linalg.norm(uo[inner])),
+                misfit_rel_err=float(abs(lg - lo) / abs(lo)),
+                grad_rel_err=float(np.linalg.norm(gg - go) / np.linalg.norm(go)))
+        log(f"accuracy vs FP64 (8 TX sample): {accuracy}")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "grad evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (fp32 MFMA) + f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": case.name, "grid": f"{ny}x{nx} (512^2 interior + 24-cell PML)", "dx_m": case.dx,
+                   "frequency_hz": f, "n_tx": ntx, "n_rx": case.n_rx, "mode": "refined", "refine_tol": args.tol},
+        "plain": plain,
+        "accuracy_vs_fp64": accuracy,
+        "phases_ms_one_step": phases,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

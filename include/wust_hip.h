@@ -174,6 +174,15 @@ int wust_directional_curvature(wust_helmholtz *h, const double *d_direction, con
  * last curvature consumer, Runtime/matlab/wust_reconstruct.m:61). */
 int wust_oracle_release(wust_helmholtz *h);
 
+/* ---- tooling: per-kernel-class HIP-event timing (no reference counterpart) ------------ */
+/* Classes: 0 assembly, 1 Schur update, 2 Gauss-Jordan panel, 3 sweep operand,
+ * 4 sweep GEMM, 5 FP64 residual, 6 source fit, 7 gradient, 8 plain factor step,
+ * 9 plain solve step.  When a class bit is set, every launch of that class is
+ * bracketed by hipEvents on its stream; wust_profile_read resolves them. */
+int wust_profile_enable(uint32_t class_mask);
+int wust_profile_read(int kernel_class, int64_t *count, double *total_ms);
+int wust_profile_reset(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,7 +80,13 @@ EXPORTS = {
     "wust_directional_curvature": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
                                              C.POINTER(C.c_double), C.c_void_p]),
     "wust_oracle_release": (C.c_int, [C.c_void_p]),
+    "wust_profile_enable": (C.c_int, [C.c_uint32]),
+    "wust_profile_read": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "wust_profile_reset": (C.c_int, []),
 }
+
+PROFILE_CLASSES = ["assembly", "schur", "gj_panel", "sweep_operand", "sweep_gemm", "residual", "source_fit",
+                   "gradient", "plain_factor_step", "plain_solve_step"]
 
 _lib = None
 

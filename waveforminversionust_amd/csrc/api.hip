@@ -10,6 +10,7 @@
 #include "engine.h"
 #include "fields.h"
 #include "plain.h"
+#include "profile.h"
 
 using namespace wust;
 
@@ -118,7 +119,7 @@ struct wust_helmholtz {
     PlainLib plain;
     int ws_Rpad = 0;
     cd *b64 = nullptr, *t64 = nullptr, *u64 = nullptr, *lam64 = nullptr;
-    cf *s32 = nullptr, *d32 = nullptr;
+    cf *s32 = nullptr, *d32 = nullptr, *b32 = nullptr;
     cf *p_src = nullptr, *p_u = nullptr, *p_tmp = nullptr;
     double *norms = nullptr;    // [2][Rpad]
     double *loss_r = nullptr;   // [Rpad]
@@ -139,6 +140,7 @@ struct wust_helmholtz {
         dfree(op.w32);
         dfree(op.pages);
         dfree(op.work);
+        dfree(op.pbuf);
         dfree(op.status);
         dfree(idx);
     }
@@ -156,6 +158,7 @@ struct wust_helmholtz {
         dfree(lam64);
         dfree(s32);
         dfree(d32);
+        dfree(b32);
         dfree(p_src);
         dfree(p_u);
         dfree(p_tmp);
@@ -180,6 +183,7 @@ struct wust_helmholtz {
         } else {
             s32 = dalloc<cf>(cnt);
             d32 = dalloc<cf>(cnt);
+            b32 = dalloc<cf>(2 * (int64_t)op.npad * Rpad);
         }
         norms = dalloc<double>(2 * (int64_t)Rpad);
         loss_r = dalloc<double>(Rpad);
@@ -254,7 +258,7 @@ struct wust_helmholtz {
             return st;
         }
         demote(b, s32, cnt, s);
-        solve_engine(op, s32, d32, Rpad, adjoint, s);
+        solve_engine(op, s32, d32, b32, Rpad, adjoint, s);
         if (op.mode == WUST_MODE_C64) {
             promote(d32, out, cnt, s);
             if (want_residual) st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr);
@@ -270,7 +274,7 @@ struct wust_helmholtz {
             if (it == 0) st.initial_residual = rel;
             st.final_residual = rel;
             if (!(rel > tol) || it >= max_iter) break;
-            solve_engine(op, s32, d32, Rpad, adjoint, s);
+            solve_engine(op, s32, d32, b32, Rpad, adjoint, s);
             u_old = bufs[cur];
             cur ^= 1;
         }
@@ -397,6 +401,7 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         const int64_t pg = (int64_t)op.npad * op.npad;
         op.pages = dalloc<cf>(pg * op.m);
         op.work = dalloc<cf>(4 * pg);
+        op.pbuf = dalloc<cf>(4 * 32 * 32);
         op.factor_bytes = (int64_t)sizeof(cf) * pg * op.m;
         WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, 4 * sizeof(uint32_t), s));
 
@@ -405,7 +410,10 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         WUST_HIP_CHECK(hipEventCreate(&e1));
         WUST_HIP_CHECK(hipEventCreate(&e2));
         WUST_HIP_CHECK(hipEventRecord(e0, s));
-        launch_assembly(ap, d_vel, d_atten, op.ck2, op.vsm, op.w64, op.w32, op.status, s);
+        {
+            ProfScope ps(PC_ASSEMBLY, s);
+            launch_assembly(ap, d_vel, d_atten, op.ck2, op.vsm, op.w64, op.w32, op.status, s);
+        }
         WUST_HIP_CHECK(hipEventRecord(e1, s));
         uint32_t st = h->read_status(s);
         if (st & ST_NONFINITE) throw Error(WUST_NUMERICAL_FAILURE, "LDR-9 produced invalid wavenumber or derivative");

@@ -22,6 +22,7 @@ struct Operator {
     cf *w32 = nullptr;
     cf *pages = nullptr;
     cf *work = nullptr;           // factor ping-pong buffers: [2 chains][2][npad*npad]
+    cf *pbuf = nullptr;           // diagonal-block inverses handed between panel launches: [2 chains][2][32*32]
     uint32_t *status = nullptr;   // device status words: [0] flags, [2..3] pivot ratios (float)
     double assembly_ms = 0, factor_ms = 0, min_pivot_ratio = 1.0;
     int64_t factor_bytes = 0;
@@ -36,7 +37,8 @@ void factor_engine(Operator &op, hipStream_t s);
 void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s);
 
 // ---- complex-FP32 block solve: out = H^{-1} src (or H^{-H} src), solve layout ----
-// src, out: [m][npad][Rpad] complex-FP32; out must not alias src.
-void solve_engine(const Operator &op, const cf *src, cf *out, int Rpad, bool adjoint, hipStream_t s);
+// src, out: [m][npad][Rpad] complex-FP32; out must not alias src; scratch: 2*npad*Rpad.
+// Rpad must be a multiple of 16.
+void solve_engine(const Operator &op, const cf *src, cf *out, cf *scratch, int Rpad, bool adjoint, hipStream_t s);
 
 }  // namespace wust

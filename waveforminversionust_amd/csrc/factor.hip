@@ -18,6 +18,7 @@
 //     WUST_SINGULAR_SCHUR_BLOCK, and the smallest pivot-to-column ratio
 //     is tracked so that a loss of dominance is visible.
 #include "engine.h"
+#include "profile.h"
 
 namespace wust {
 
@@ -81,126 +82,208 @@ __global__ void __launch_bounds__(256) schur_kernel(SchurParams p) {
 struct GjTask {
     const cf *src;
     cf *dst;
+    const cf *pin;   // inverse of this panel's diagonal block (32x32, ld 32), or null
+    cf *pout;        // where the next panel's diagonal-block inverse goes, or null
 };
 struct GjParams {
     GjTask task[2];
     int ntasks, npad, q;
     uint32_t *status;
-    float *pivot_ratio;   // per task: running min |pivot| / max |column entry of the diagonal block|
+    float *pivot_ratio;   // per task: running min |pivot| / max |entry below it| in the diagonal block
 };
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct GjShared {
+    cf Pd[NB][NB + 1];     // P, row-major [r][c]
+    cf PT[NB][NB + 1];     // P transposed [c][r]
+    cf AikT[NB][NB + 1];   // A_IK transposed [l][i]; later W^T
+    cf Akj[NB][NB + 1];    // A_KJ row-major [l][j]
+    cf rowb[2][NB], colb[2][NB];
+    float colmag[NB][NB];  // |column entries below the pivot|^2 (monitoring WG only)
+    cf pivlog[NB];
+};
+
+// In-place pivot-free Gauss-Jordan inverse of a 32x32 block held by 256
+// threads, thread t owning elements (t & 31, (t >> 5) + 8e), e = 0..3.  One
+// LDS-only barrier per pivot; pivot row/column double-buffered by parity; the
+// update is branch-free.  With `monitor`, zero/non-finite pivots and the
+// smallest |pivot| / max|entry below it| are reported.
+__device__ __forceinline__ void gj_invert32(cf (&a)[4], GjShared &sh, bool monitor, uint32_t *status,
+                                            float *ratio_slot) {
+    const int t = threadIdx.x, i = t & 31, jb = t >> 5;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int pb = k & 1;
+        if (i == k) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sh.rowb[pb][jb + 8 * e] = a[e];
+        }
+        if (jb == (k & 7)) {
+            sh.colb[pb][i] = a[k >> 3];
+            if (monitor) sh.colmag[k][i] = (i > k) ? abs2(a[k >> 3]) : 0.f;
+        }
+        lds_barrier();
+        const cf pkk = sh.rowb[pb][k];
+        const cf cik = sh.colb[pb][i];
+        cf rk[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rk[e] = sh.rowb[pb][jb + 8 * e];
+        const cf piv = crecip_fast(pkk);
+        const cf ck = cik * piv;
+        const cf nck = -ck;
+        if (monitor && t == 0) sh.pivlog[k] = pkk;
+        const bool rowk = (i == k);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const bool colk = (jb + 8 * e == k);
+            const cf g = a[e] - ck * rk[e];
+            const cf rv = rk[e] * piv;
+            const cf onrow = colk ? piv : rv;
+            const cf offrow = colk ? nck : g;
+            a[e] = rowk ? onrow : offrow;
+        }
+    }
+    if (monitor) {
+        lds_barrier();
+        if (t < NB) {
+            float mx = 0.f;
+            for (int z = 0; z < NB; ++z) mx = fmaxf(mx, sh.colmag[t][z]);
+            const float a2 = abs2(sh.pivlog[t]);
+            float r = (mx > 0.f) ? sqrtf(a2 / mx) : 1.0f;
+            if (!(a2 > 0.f)) atomicOr(status, (uint32_t)ST_ZERO_PIVOT);
+            if (!isfinite(a2)) atomicOr(status, (uint32_t)ST_NONFINITE_PIVOT);
+            for (int off = 16; off > 0; off >>= 1) r = fminf(r, __shfl_xor(r, off, 64));
+            if (t == 0 && r < 1.0f) atomicMin(reinterpret_cast<int *>(ratio_slot), __float_as_int(r));
+        }
+    }
+}
+
+// D^T quadrant (16 x 16) of D = A B over a 32-deep reduction, computed as
+// B^T A^T on MFMA so that lane (li, kq) ends up holding D[row0 + li][col0 + 4 kq + r]
+// (contiguous rows for a fixed column: column-major stores coalesce).
+//   Bt(k, j): element B[k][col0 + j]   (LDS, k-major)
+//   At(k, i): element A[row0 + i][k]   (LDS, k-major)
+template <typename FB, typename FA>
+__device__ __forceinline__ void mfma_quadrant(v4f &dr, v4f &di, FB Bt, FA At, int li, int kq) {
+#pragma unroll
+    for (int ks = 0; ks < NB; ks += 4) {
+        const cf a = Bt(ks + kq, li);   // "A" operand of the MFMA: row = column index j of D
+        const cf b = At(ks + kq, li);   // "B" operand: column = row index i of D
+        dr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.re, b.re, dr, 0, 0, 0);
+        dr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.im, b.im, dr, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_16x16x4f32(a.re, b.im, di, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_16x16x4f32(a.im, b.re, di, 0, 0, 0);
+    }
+}
 
 // One Gauss-Jordan panel step on a 32x32 output tile.  With K the panel block
 // and P = A_KK^{-1}:  A_IJ <- A_IJ - A_IK P A_KJ,  A_IK <- -A_IK P,
 // A_KJ <- P A_KJ,  A_KK <- P.  Reads src, writes dst (no in-place hazards).
+// P comes from the previous launch (pin) when available; otherwise every WG
+// inverts A_KK itself.  The WG that produces the next panel's diagonal tile
+// also inverts it and publishes the inverse (pout), so the serial pivot chain
+// of the next panel runs in one WG instead of all of them.
 __global__ void __launch_bounds__(256) gj_panel_kernel(GjParams p) {
     const int T = p.npad / NB;
-    int b = blockIdx.x;
-    int task = b / (T * T);
-    int rem = b - task * T * T;
-    int tj = rem / T, ti = rem - tj * T;
+    const int b = blockIdx.x;
+    const int task = b / (T * T);
+    const int rem = b - task * T * T;
+    const int tj = rem / T, ti = rem - tj * T;
     const GjTask &tk = p.task[task];
-    const int ld = p.npad;
+    const int64_t ld = p.npad;
     const int q = p.q;
-    __shared__ cf Ps[NB][NB + 1];   // [col][row]
-    __shared__ cf Ik[NB][NB + 1];   // A_IK, then W = A_IK P
-    __shared__ cf Kj[NB][NB + 1];   // A_KJ
-    __shared__ float colmax[NB];
     const int t = threadIdx.x;
-    const int i = t & 31, jb = t >> 5;
+    const int i = t & 31, jb = t >> 5;           // element ownership: (i, jb + 8e)
+    const int lane = t & 63, wave = t >> 6;
+    const int li = lane & 15, kq = lane >> 4;
+    const int qr = wave & 1, qc = wave >> 1;     // output quadrant of this wave
+    __shared__ GjShared sh;
 
     const cf *src = tk.src;
-    // load the diagonal block and the strips this tile needs
-    for (int r = 0; r < 4; ++r) {
-        int j = jb + 8 * r;
-        Ps[j][i] = src[(int64_t)(q * NB + i) + (int64_t)ld * (q * NB + j)];
-        if (ti != q) Ik[j][i] = src[(int64_t)(ti * NB + i) + (int64_t)ld * (q * NB + j)];
-        if (tj != q) Kj[j][i] = src[(int64_t)(q * NB + i) + (int64_t)ld * (tj * NB + j)];
+    const bool rowK = (ti == q), colK = (tj == q);
+    // issue every global load first; they complete while the pivots run
+    cf a[4], aik[4], akj[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int j = jb + 8 * e;
+        a[e] = tk.pin ? tk.pin[i + NB * j] : src[(int64_t)(q * NB + i) + ld * (q * NB + j)];
+        aik[e] = rowK ? mk(0.f, 0.f) : src[(int64_t)(ti * NB + i) + ld * (q * NB + j)];
+        akj[e] = colK ? mk(0.f, 0.f) : src[(int64_t)(q * NB + i) + ld * (tj * NB + j)];
     }
     cf aij[4];
-    if (ti != q && tj != q)
-        for (int r = 0; r < 4; ++r) {
-            int j = jb + 8 * r;
-            aij[r] = src[(int64_t)(ti * NB + i) + (int64_t)ld * (tj * NB + j)];
-        }
-    __syncthreads();
-
-    // invert the diagonal block in place (pivot-free Gauss-Jordan)
-    float ratio = 1.0f;
-    for (int k = 0; k < NB; ++k) {
-        cf pkk = Ps[k][k];
-        if (t < 32) {
-            // column-dominance monitor inside the diagonal block
-            float v = (i > k) ? abs2(Ps[k][i]) : 0.f;
-            colmax[i] = v;
-        }
-        cf piv = crecip(pkk);
-        cf nv[4];
-        for (int r = 0; r < 4; ++r) {
-            int j = jb + 8 * r;
-            cf aik = Ps[k][i], akj = Ps[j][k], a = Ps[j][i];
-            if (i == k && j == k) nv[r] = piv;
-            else if (i == k) nv[r] = akj * piv;
-            else if (j == k) nv[r] = -(aik * piv);
-            else nv[r] = a - aik * (akj * piv);
-        }
-        __syncthreads();
-        if (t == 0) {
-            float a2 = abs2(pkk);
-            if (!(a2 > 0.f)) atomicOr(p.status, (uint32_t)ST_ZERO_PIVOT);
-            if (!isfinite(a2)) atomicOr(p.status, (uint32_t)ST_NONFINITE_PIVOT);
-            float cm = 0.f;
-            for (int z = 0; z < NB; ++z) cm = fmaxf(cm, colmax[z]);
-            if (cm > 0.f) ratio = fminf(ratio, sqrtf(a2 / cm));
-        }
-        for (int r = 0; r < 4; ++r) Ps[jb + 8 * r][i] = nv[r];
-        __syncthreads();
-    }
-    if (t == 0 && ti == q && tj == q && ratio < 1.0f) {
-        // store min ratio (positive floats order like their bit patterns)
-        atomicMin(reinterpret_cast<int *>(p.pivot_ratio + task), __float_as_int(ratio));
-    }
-
-    cf out[4];
-    if (ti == q && tj == q) {
-        for (int r = 0; r < 4; ++r) out[r] = Ps[jb + 8 * r][i];
-    } else if (ti == q) {
-        // P A_KJ
-        for (int r = 0; r < 4; ++r) {
-            int j = jb + 8 * r;
-            cf acc = mk(0.f, 0.f);
-            for (int l = 0; l < NB; ++l) cfma(acc, Ps[l][i], Kj[j][l]);
-            out[r] = acc;
-        }
-    } else {
-        // W = A_IK P
-        cf w[4];
-        for (int r = 0; r < 4; ++r) {
-            int j = jb + 8 * r;
-            cf acc = mk(0.f, 0.f);
-            for (int l = 0; l < NB; ++l) cfma(acc, Ik[l][i], Ps[j][l]);
-            w[r] = acc;
-        }
-        if (tj == q) {
-            for (int r = 0; r < 4; ++r) out[r] = -w[r];
-        } else {
-            __syncthreads();
-            for (int r = 0; r < 4; ++r) Ik[jb + 8 * r][i] = w[r];
-            __syncthreads();
-            for (int r = 0; r < 4; ++r) {
-                int j = jb + 8 * r;
-                cf acc = aij[r];
-                for (int l = 0; l < NB; ++l) {
-                    cf wl = Ik[l][i], kl = Kj[j][l];
-                    acc.re -= wl.re * kl.re - wl.im * kl.im;
-                    acc.im -= wl.re * kl.im + wl.im * kl.re;
-                }
-                out[r] = acc;
-            }
-        }
-    }
+    const bool general = !rowK && !colK;
+#pragma unroll
     for (int r = 0; r < 4; ++r) {
-        int j = jb + 8 * r;
-        tk.dst[(int64_t)(ti * NB + i) + (int64_t)ld * (tj * NB + j)] = out[r];
+        const int row = ti * NB + qr * 16 + li, col = tj * NB + qc * 16 + 4 * kq + r;
+        aij[r] = general ? src[(int64_t)row + ld * col] : mk(0.f, 0.f);
+    }
+    if (!tk.pin) gj_invert32(a, sh, rowK && colK, p.status, p.pivot_ratio + task);
+
+    // stage P and the strips for the MFMA products
+    lds_barrier();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int j = jb + 8 * e;
+        sh.Pd[i][j] = a[e];
+        sh.PT[j][i] = a[e];
+        sh.AikT[j][i] = aik[e];   // element (i, l = j) of A_IK
+        sh.Akj[i][j] = akj[e];    // element (l = i, j) of A_KJ
+    }
+    lds_barrier();
+
+    v4f dr = {0.f, 0.f, 0.f, 0.f}, di = dr;
+    cf out[4];
+    if (rowK && colK) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[r] = sh.Pd[qr * 16 + li][qc * 16 + 4 * kq + r];
+    } else if (rowK) {
+        // P A_KJ
+        mfma_quadrant(dr, di, [&](int k, int j) { return sh.Akj[k][qc * 16 + j]; },
+                      [&](int k, int ii) { return sh.PT[k][qr * 16 + ii]; }, li, kq);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[r] = mk(dr[r], di[r]);
+    } else if (colK) {
+        // -(A_IK P)
+        mfma_quadrant(dr, di, [&](int k, int j) { return sh.Pd[k][qc * 16 + j]; },
+                      [&](int k, int ii) { return sh.AikT[k][qr * 16 + ii]; }, li, kq);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[r] = mk(-dr[r], -di[r]);
+    } else {
+        // W = A_IK P (quadrant), shared through LDS as W^T, then A_IJ - W A_KJ
+        mfma_quadrant(dr, di, [&](int k, int j) { return sh.Pd[k][qc * 16 + j]; },
+                      [&](int k, int ii) { return sh.AikT[k][qr * 16 + ii]; }, li, kq);
+        lds_barrier();   // every wave is done reading AikT
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sh.AikT[qc * 16 + 4 * kq + r][qr * 16 + li] = mk(dr[r], di[r]);
+        lds_barrier();
+        v4f er = {0.f, 0.f, 0.f, 0.f}, ei = er;
+        mfma_quadrant(er, ei, [&](int k, int j) { return sh.Akj[k][qc * 16 + j]; },
+                      [&](int k, int ii) { return sh.AikT[k][qr * 16 + ii]; }, li, kq);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[r] = mk(aij[r].re - er[r], aij[r].im - ei[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = ti * NB + qr * 16 + li, col = tj * NB + qc * 16 + 4 * kq + r;
+        tk.dst[(int64_t)row + ld * col] = out[r];
+    }
+    // the next panel's diagonal tile: invert it here and publish the inverse
+    if (tk.pout && ti == q + 1 && tj == q + 1) {
+        // the serial pivot chain is the launch's critical path: let its waves win
+        // issue arbitration against the other WGs sharing this CU
+        __builtin_amdgcn_s_setprio(3);
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sh.Pd[qr * 16 + li][qc * 16 + 4 * kq + r] = out[r];
+        lds_barrier();
+        cf nx[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) nx[e] = sh.Pd[i][jb + 8 * e];
+        gj_invert32(nx, sh, true, p.status, p.pivot_ratio + task);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tk.pout[i + NB * (jb + 8 * e)] = nx[e];
+        __builtin_amdgcn_s_setprio(0);
     }
 }
 
@@ -269,8 +352,11 @@ void factor_engine(Operator &op, hipStream_t s) {
                 add_term(op, tk.t[tk.nterms++], op.page(j + 1), j, Udirs, j + 1, Ldirs);
         }
         dim3 grid((ld + 255) / 256, ld, sp.ntasks);
-        hipLaunchKernelGGL(schur_kernel, grid, dim3(256), 0, s, sp);
-        WUST_LAUNCH_CHECK();
+        {
+            ProfScope ps(PC_SCHUR, s);
+            hipLaunchKernelGGL(schur_kernel, grid, dim3(256), 0, s, sp);
+            WUST_LAUNCH_CHECK();
+        }
         const int Q = ld / NB;
         for (int q = 0; q < Q; ++q) {
             GjParams gp{};
@@ -284,7 +370,11 @@ void factor_engine(Operator &op, hipStream_t s) {
                 cf *b1 = b0 + pg;
                 gp.task[c].src = (q % 2 == 0) ? b0 : b1;
                 gp.task[c].dst = (q == Q - 1) ? op.page(steps[c].block) : ((q % 2 == 0) ? b1 : b0);
+                cf *pb = op.pbuf + (int64_t)c * 2 * NB * NB;
+                gp.task[c].pin = (q == 0) ? nullptr : pb + (q % 2) * NB * NB;
+                gp.task[c].pout = (q + 1 < Q) ? pb + ((q + 1) % 2) * NB * NB : nullptr;
             }
+            ProfScope ps(PC_GJ_PANEL, s);
             hipLaunchKernelGGL(gj_panel_kernel, dim3(sp.ntasks * (ld / NB) * (ld / NB)), dim3(256), 0, s, gp);
             WUST_LAUNCH_CHECK();
         }

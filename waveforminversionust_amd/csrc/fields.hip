@@ -5,6 +5,7 @@
 // index of every solve-layout array so that the stencil neighbours, the
 // coefficient rows and the stores are coalesced.
 #include "fields.h"
+#include "profile.h"
 
 namespace wust {
 
@@ -344,6 +345,7 @@ void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int 
     WUST_LAUNCH_CHECK();
 }
 void launch_residual(const ResidualArgs &a, hipStream_t s) {
+    ProfScope ps(PC_RESIDUAL, s);
     dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);
     hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, s, a);
     WUST_LAUNCH_CHECK();
@@ -361,10 +363,12 @@ void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int R
     WUST_LAUNCH_CHECK();
 }
 void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s) {
+    ProfScope ps(PC_SOURCE_FIT, s);
     hipLaunchKernelGGL(source_fit_kernel, dim3(R), dim3(256), 0, s, a);
     WUST_LAUNCH_CHECK();
 }
 void launch_gradient(const GradientArgs &a, hipStream_t s) {
+    ProfScope ps(PC_GRADIENT, s);
     dim3 grid((a.n + 3) / 4, a.m);
     hipLaunchKernelGGL(gradient_kernel, grid, dim3(256), 0, s, a);
     WUST_LAUNCH_CHECK();

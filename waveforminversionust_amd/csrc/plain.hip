@@ -13,6 +13,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include "plain.h"
+#include "profile.h"
 
 namespace wust {
 
@@ -108,6 +109,7 @@ void factor_plain(Operator &op, PlainLib &lib, hipStream_t s) {
     cf *T = op.work;
     WUST_HIP_CHECK(hipMemsetAsync(lib.info, 0, sizeof(int) * 2 * (size_t)(m < 65536 ? m : 65536), s));
     for (int j = 0; j < m; ++j) {
+        ProfScope ps(PC_PLAIN_FACTOR_STEP, s);
         launch_schur_left(op, j, T, s);
         ROCBLAS_CHECK(rocsolver_cgetrf(h, n, n, (rocblas_float_complex *)T, ld, lib.ipiv, lib.info + 2 * (j % 65536)));
         cf *X = op.page(j);
@@ -149,12 +151,14 @@ void solve_plain(const Operator &op, PlainLib &lib, const cf *src, cf *u, cf *tm
     int Ud[3] = {DNE, DE, DSE};
     // forward pass: u_j = X_j (b_j - C_j u_{j-1})  (C = L_j, or U_{j-1}^H for the adjoint)
     for (int j = 0; j < m; ++j) {
+        ProfScope ps(PC_PLAIN_SOLVE_STEP, s);
         if (!adjoint) coupling(j, true, Ld, j, j - 1, j >= 1, false);
         else coupling(j, true, Ud, j - 1, j - 1, j >= 1, true);
         gemm(j, &one, &zero);
     }
     // backward pass: u_j -= X_j (C'_j u_{j+1})  (C' = U_j, or L_{j+1}^H)
     for (int j = m - 2; j >= 0; --j) {
+        ProfScope ps(PC_PLAIN_SOLVE_STEP, s);
         if (!adjoint) coupling(j, false, Ud, j, j + 1, true, false);
         else coupling(j, false, Ld, j + 1, j + 1, true, true);
         // tmp holds -(C' u): coupling() computes 0 - C'u

@@ -44,6 +44,27 @@ __host__ __device__ __forceinline__ cf crecip(cf a) {
     return {a.re * s, -a.im * s};
 }
 
+// Complex reciprocal with the hardware v_rcp_f32 (1 ulp); used on latency-critical
+// pivot chains whose result is corrected by FP64 refinement.
+__device__ __forceinline__ cf crecip_fast(cf a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    float s = __builtin_amdgcn_rcpf(a.re * a.re + a.im * a.im);
+#else
+    float s = 1.0f / (a.re * a.re + a.im * a.im);
+#endif
+    return {a.re * s, -a.im * s};
+}
+
+// Workgroup barrier that orders LDS only: it waits for this wave's LDS
+// operations (lgkmcnt) but not for global loads still in flight, which the
+// HIP __syncthreads() fence would drain.  The "memory" clobber keeps the
+// compiler from moving shared-memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+#endif
+}
+
 __host__ __device__ __forceinline__ cd operator+(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
 __host__ __device__ __forceinline__ cd operator-(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
 __host__ __device__ __forceinline__ cd operator-(cd a) { return {-a.re, -a.im}; }
