@@ -320,8 +320,7 @@ void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s) {
     tk.T = T;
     if (j >= 1) add_term(op, tk.t[tk.nterms++], op.page(j - 1), j, Ldirs, j - 1, Udirs);
     dim3 grid((op.npad + 255) / 256, op.npad, 1);
-    hipLaunchKernelGGL(schur_kernel, grid, dim3(256), 0, s, sp);
-    WUST_LAUNCH_CHECK();
+    launch_k(PC_SCHUR, schur_kernel, grid, dim3(256), 0, s, sp);
 }
 
 void factor_engine(Operator &op, hipStream_t s) {
@@ -352,11 +351,7 @@ void factor_engine(Operator &op, hipStream_t s) {
                 add_term(op, tk.t[tk.nterms++], op.page(j + 1), j, Udirs, j + 1, Ldirs);
         }
         dim3 grid((ld + 255) / 256, ld, sp.ntasks);
-        {
-            ProfScope ps(PC_SCHUR, s);
-            hipLaunchKernelGGL(schur_kernel, grid, dim3(256), 0, s, sp);
-            WUST_LAUNCH_CHECK();
-        }
+        launch_k(PC_SCHUR, schur_kernel, grid, dim3(256), 0, s, sp);
         const int Q = ld / NB;
         for (int q = 0; q < Q; ++q) {
             GjParams gp{};
@@ -374,9 +369,7 @@ void factor_engine(Operator &op, hipStream_t s) {
                 gp.task[c].pin = (q == 0) ? nullptr : pb + (q % 2) * NB * NB;
                 gp.task[c].pout = (q + 1 < Q) ? pb + ((q + 1) % 2) * NB * NB : nullptr;
             }
-            ProfScope ps(PC_GJ_PANEL, s);
-            hipLaunchKernelGGL(gj_panel_kernel, dim3(sp.ntasks * (ld / NB) * (ld / NB)), dim3(256), 0, s, gp);
-            WUST_LAUNCH_CHECK();
+            launch_k(PC_GJ_PANEL, gj_panel_kernel, dim3(sp.ntasks * (ld / NB) * (ld / NB)), dim3(256), 0, s, gp);
         }
     };
 

@@ -345,10 +345,8 @@ void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int 
     WUST_LAUNCH_CHECK();
 }
 void launch_residual(const ResidualArgs &a, hipStream_t s) {
-    ProfScope ps(PC_RESIDUAL, s);
     dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);
-    hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, s, a);
-    WUST_LAUNCH_CHECK();
+    launch_k(PC_RESIDUAL, residual_kernel, grid, dim3(256), 0, s, a);
 }
 void promote(const cf *in, cd *out, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(promote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
@@ -363,15 +361,11 @@ void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int R
     WUST_LAUNCH_CHECK();
 }
 void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s) {
-    ProfScope ps(PC_SOURCE_FIT, s);
-    hipLaunchKernelGGL(source_fit_kernel, dim3(R), dim3(256), 0, s, a);
-    WUST_LAUNCH_CHECK();
+    launch_k(PC_SOURCE_FIT, source_fit_kernel, dim3(R), dim3(256), 0, s, a);
 }
 void launch_gradient(const GradientArgs &a, hipStream_t s) {
-    ProfScope ps(PC_GRADIENT, s);
     dim3 grid((a.n + 3) / 4, a.m);
-    hipLaunchKernelGGL(gradient_kernel, grid, dim3(256), 0, s, a);
-    WUST_LAUNCH_CHECK();
+    launch_k(PC_GRADIENT, gradient_kernel, grid, dim3(256), 0, s, a);
 }
 void launch_curvature_source(const CurvSrcArgs &a, hipStream_t s) {
     dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);

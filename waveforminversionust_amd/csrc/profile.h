@@ -1,5 +1,7 @@
 // Per-kernel-class HIP-event timing (tooling; off unless enabled).
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "wust_common.h"
 
 namespace wust {
@@ -21,6 +23,23 @@ enum ProfClass {
 bool prof_on(int cls);
 void prof_record(int cls, bool begin, hipStream_t s);
 void prof_flush();
+hipEvent_t prof_event();
+void prof_push(int cls, hipEvent_t a, hipEvent_t b);
+
+// Launch a kernel; when its class is profiled, the dispatch packet itself
+// timestamps start and end (hipExtLaunchKernelGGL events), so the timing adds
+// no packets to the stream.
+template <typename K, typename... Args>
+void launch_k(int cls, K kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s, Args... args) {
+    if (prof_on(cls)) {
+        hipEvent_t a = prof_event(), b = prof_event();
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, a, b, 0, args...);
+        prof_push(cls, a, b);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+    }
+    WUST_LAUNCH_CHECK();
+}
 
 // RAII bracket around one launch (or a fixed group of launches) of a class.
 struct ProfScope {

@@ -56,6 +56,18 @@ void prof_record(int cls, bool begin, hipStream_t s) {
     }
 }
 
+hipEvent_t prof_event() {
+    Profiler &p = P();
+    std::lock_guard<std::mutex> lk(p.mu);
+    return p.get();
+}
+
+void prof_push(int cls, hipEvent_t a, hipEvent_t b) {
+    Profiler &p = P();
+    std::lock_guard<std::mutex> lk(p.mu);
+    p.pending.push_back({cls, a, b});
+}
+
 void prof_flush() {
     Profiler &p = P();
     std::lock_guard<std::mutex> lk(p.mu);

@@ -198,16 +198,10 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *scratch, int R
         sp.npad = npad;
         sp.Rpad = Rpad;
         const int64_t nops = (int64_t)sp.ntasks * blk;
-        {
-            ProfScope ps(PC_SWEEP_OPERAND, s);
-            hipLaunchKernelGGL(sweep_operand_kernel, dim3((unsigned)((nops + 255) / 256)), dim3(256), 0, s, sp);
-            WUST_LAUNCH_CHECK();
-        }
+        launch_k(PC_SWEEP_OPERAND, sweep_operand_kernel, dim3((unsigned)((nops + 255) / 256)), dim3(256), 0, s, sp);
         sp.units = sp.ntasks * (npad / 16) * (Rpad / 16);
         sp.per_xcd = (sp.units + 7) / 8;
-        ProfScope ps(PC_SWEEP_GEMM, s);
-        hipLaunchKernelGGL(sweep_gemm_kernel, dim3(8 * sp.per_xcd), dim3(256), 0, s, sp);
-        WUST_LAUNCH_CHECK();
+        launch_k(PC_SWEEP_GEMM, sweep_gemm_kernel, dim3(8 * sp.per_xcd), dim3(256), 0, s, sp);
     };
     auto base = [&](int j, bool with_src, bool acc) {
         SweepTask tk{};
