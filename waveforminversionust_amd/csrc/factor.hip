@@ -10,13 +10,20 @@
 //     halves and every step carries two independent inversions;
 //   * only the interior (n = Ny-2) rows of each column are kept, which is what
 //     the reference's interior-only getrf/getrs computes;
-//   * each block is inverted in place by blocked Gauss-Jordan elimination
-//     (one launch per 32-column panel, ping-ponging between two buffers).
+//   * each block is inverted by blocked Gauss-Jordan elimination (one launch
+//     per 32-column panel, ping-ponging between two buffers); the 32x32
+//     diagonal-block inverses on the critical path use a 2x2 split into 16x16
+//     single-wave register-resident inversions and MFMA products, and the WG
+//     that produces the next panel's diagonal tile inverts it for the next
+//     launch.
 //     The Schur blocks of the nine-point PML operator are column-diagonally
 //     dominant (partial pivoting selects the diagonal at every step), so the
 //     elimination is pivot-free; a zero or non-finite pivot is reported as
-//     WUST_SINGULAR_SCHUR_BLOCK, and the smallest pivot-to-column ratio
-//     is tracked so that a loss of dominance is visible.
+//     WUST_SINGULAR_SCHUR_BLOCK, and the smallest |pivot| / max|entry below
+//     it| inside each 16x16 pivot block is tracked so that a loss of
+//     dominance is visible.
+#include <utility>
+
 #include "engine.h"
 #include "profile.h"
 
@@ -99,64 +106,152 @@ struct GjShared {
     cf PT[NB][NB + 1];     // P transposed [c][r]
     cf AikT[NB][NB + 1];   // A_IK transposed [l][i]; later W^T
     cf Akj[NB][NB + 1];    // A_KJ row-major [l][j]
-    cf rowb[2][NB], colb[2][NB];
-    float colmag[NB][NB];  // |column entries below the pivot|^2 (monitoring WG only)
-    cf pivlog[NB];
 };
 
-// In-place pivot-free Gauss-Jordan inverse of a 32x32 block held by 256
-// threads, thread t owning elements (t & 31, (t >> 5) + 8e), e = 0..3.  One
-// LDS-only barrier per pivot; pivot row/column double-buffered by parity; the
-// update is branch-free.  With `monitor`, zero/non-finite pivots and the
-// smallest |pivot| / max|entry below it| are reported.
-__device__ __forceinline__ void gj_invert32(cf (&a)[4], GjShared &sh, bool monitor, uint32_t *status,
-                                            float *ratio_slot) {
-    const int t = threadIdx.x, i = t & 31, jb = t >> 5;
+// ---------------------------------------------------------------------------
+// 32x32 inverse by 2x2 blocks of 16 (the serial chain of the factor):
+//   Ai = A^-1;  E = C Ai;  F = Ai B;  S = D - E B;  Si = S^-1;
+//   X = [[Ai - F X21, -F Si], [X21 = -Si E, Si]]
+// Two 16-pivot Gauss-Jordan inversions, each by one wave with the block in
+// registers, and 16x16 complex products on MFMA, each product on one wave.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_lds_sync() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#endif
+}
+
+__device__ __forceinline__ float bits_f(int v) { return __int_as_float(v); }
+__device__ __forceinline__ int f_bits(float v) { return __float_as_int(v); }
+
+// In-place pivot-free Gauss-Jordan inverse of the row-major 16x16 block M (ld)
+// by ONE wave with the block in registers: lane (r = lane & 15, cg = lane >> 4)
+// holds a[e] = M[r][cg + 4e].  Per pivot k the three operands arrive without
+// an LDS round trip or barrier:
+//   pivot M[k][k]        readlane of lane (k, k & 3)                (scalar)
+//   row k, my columns    DPP row_newbcast:k inside my 16-lane row
+//   column k, my row     ds_bpermute from lane (r, k & 3)
+// With `monitor`, reports zero / non-finite pivots and min |pivot| / max
+// |entry below it in its column|; the monitor keeps per-lane state (the lanes
+// on column k below the pivot compare |pivot|^2 / |own entry|^2 with their best
+// pair by cross multiplication) and reduces once, after the last pivot, so the
+// serial pivot chain carries no shuffles, divides or roots.
+struct Inv16Mon {
+    float best_num = 1.f, best_den = 1.f, pmin = 1.f, pchk = 0.f;
+};
+
+template <int K>
+__device__ __forceinline__ void inv16_reg_pivot(cf (&a)[4], int r, int cg, bool monitor, Inv16Mon &mon) {
+    constexpr int kc = K & 3, ke = K >> 2;
+    const cf pk = mk(bits_f(__builtin_amdgcn_readlane(f_bits(a[ke].re), kc * 16 + K)),
+                     bits_f(__builtin_amdgcn_readlane(f_bits(a[ke].im), kc * 16 + K)));
+    const int src = (kc * 16 + r) * 4;
+    const cf ck = mk(bits_f(__builtin_amdgcn_ds_bpermute(src, f_bits(a[ke].re))),
+                     bits_f(__builtin_amdgcn_ds_bpermute(src, f_bits(a[ke].im))));
+    cf rk[4];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        const int pb = k & 1;
-        if (i == k) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) sh.rowb[pb][jb + 8 * e] = a[e];
-        }
-        if (jb == (k & 7)) {
-            sh.colb[pb][i] = a[k >> 3];
-            if (monitor) sh.colmag[k][i] = (i > k) ? abs2(a[k >> 3]) : 0.f;
-        }
-        lds_barrier();
-        const cf pkk = sh.rowb[pb][k];
-        const cf cik = sh.colb[pb][i];
-        cf rk[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) rk[e] = sh.rowb[pb][jb + 8 * e];
-        const cf piv = crecip_fast(pkk);
-        const cf ck = cik * piv;
-        const cf nck = -ck;
-        if (monitor && t == 0) sh.pivlog[k] = pkk;
-        const bool rowk = (i == k);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const bool colk = (jb + 8 * e == k);
-            const cf g = a[e] - ck * rk[e];
-            const cf rv = rk[e] * piv;
-            const cf onrow = colk ? piv : rv;
-            const cf offrow = colk ? nck : g;
-            a[e] = rowk ? onrow : offrow;
-        }
-    }
+    for (int e = 0; e < 4; ++e)
+        rk[e] = mk(bits_f(__builtin_amdgcn_update_dpp(0, f_bits(a[e].re), 0x150 + K, 0xf, 0xf, false)),
+                   bits_f(__builtin_amdgcn_update_dpp(0, f_bits(a[e].im), 0x150 + K, 0xf, 0xf, false)));
     if (monitor) {
-        lds_barrier();
-        if (t < NB) {
-            float mx = 0.f;
-            for (int z = 0; z < NB; ++z) mx = fmaxf(mx, sh.colmag[t][z]);
-            const float a2 = abs2(sh.pivlog[t]);
-            float r = (mx > 0.f) ? sqrtf(a2 / mx) : 1.0f;
-            if (!(a2 > 0.f)) atomicOr(status, (uint32_t)ST_ZERO_PIVOT);
-            if (!isfinite(a2)) atomicOr(status, (uint32_t)ST_NONFINITE_PIVOT);
-            for (int off = 16; off > 0; off >>= 1) r = fminf(r, __shfl_xor(r, off, 64));
-            if (t == 0 && r < 1.0f) atomicMin(reinterpret_cast<int *>(ratio_slot), __float_as_int(r));
+        const float a2 = abs2(pk);
+        mon.pmin = fminf(mon.pmin, a2);
+        mon.pchk += a2 * 0.f;
+        const float e2 = abs2(a[ke]);
+        const bool cand = (cg == kc) && (r > K) && (a2 * mon.best_den < mon.best_num * e2);
+        mon.best_num = cand ? a2 : mon.best_num;
+        mon.best_den = cand ? e2 : mon.best_den;
+    }
+    const cf piv = crecip_fast(pk);
+    const cf ckp = ck * piv;
+    const bool rowk = (r == K);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const bool colk = (e == ke) && (cg == kc);
+        const cf g = a[e] - ckp * rk[e];
+        const cf onrow = colk ? piv : rk[e] * piv;
+        const cf offrow = colk ? -ckp : g;
+        a[e] = rowk ? onrow : offrow;
+    }
+}
+
+template <int... K>
+__device__ __forceinline__ void inv16_reg_pivots(cf (&a)[4], int r, int cg, bool monitor, Inv16Mon &mon,
+                                                 std::integer_sequence<int, K...>) {
+    (inv16_reg_pivot<K>(a, r, cg, monitor, mon), ...);
+}
+
+__device__ __forceinline__ void inv16_reg(cf *M, int ld, bool monitor, uint32_t *status, float *ratio_slot) {
+    const int lane = threadIdx.x & 63, r = lane & 15, cg = lane >> 4;
+    cf a[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) a[e] = M[r * ld + cg + 4 * e];
+    Inv16Mon mon;
+    inv16_reg_pivots(a, r, cg, monitor, mon, std::make_integer_sequence<int, 16>{});
+#pragma unroll
+    for (int e = 0; e < 4; ++e) M[r * ld + cg + 4 * e] = a[e];
+    wave_lds_sync();
+    if (monitor) {
+        float ratio = (mon.best_den > 0.f) ? sqrtf(mon.best_num / mon.best_den) : 1.f;
+        bool bad = !(mon.pmin > 0.f) || (mon.pchk != mon.pchk);
+        for (int off = 32; off > 0; off >>= 1) ratio = fminf(ratio, __shfl_xor(ratio, off, 64));
+        bad = __any(bad);
+        if (lane == 0) {
+            if (bad) atomicOr(status, (uint32_t)ST_ZERO_PIVOT);
+            if (ratio < 1.0f) atomicMin(reinterpret_cast<int *>(ratio_slot), __float_as_int(ratio));
         }
     }
+}
+
+// C = sgn * (A B) + (Cin ? Cin : 0) for row-major 16x16 blocks (ld), one wave.
+// Cin may alias C (each lane reads the elements it later writes).
+__device__ __forceinline__ void gemm16_wave(const cf *A, const cf *B, const cf *Cin, cf *C, int ld, float sgn) {
+    const int lane = threadIdx.x & 63, li = lane & 15, kq = lane >> 4;
+    v4f dr = {0.f, 0.f, 0.f, 0.f}, di = dr;
+#pragma unroll
+    for (int ks = 0; ks < 16; ks += 4) {
+        const cf av = A[li * ld + ks + kq];   // MFMA A operand: row li, k = ks + kq
+        const cf bv = B[(ks + kq) * ld + li]; // MFMA B operand: k, column li
+        dr = __builtin_amdgcn_mfma_f32_16x16x4f32(av.re, bv.re, dr, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_16x16x4f32(av.re, bv.im, di, 0, 0, 0);
+        dr = __builtin_amdgcn_mfma_f32_16x16x4f32(-av.im, bv.im, dr, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f32_16x16x4f32(av.im, bv.re, di, 0, 0, 0);
+    }
+    // lane holds rows 4 kq + r, column li
+    cf out[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        out[r] = mk(sgn * dr[r], sgn * di[r]);
+        if (Cin) out[r] = out[r] + Cin[(4 * kq + r) * ld + li];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) C[(4 * kq + r) * ld + li] = out[r];
+    wave_lds_sync();
+}
+
+// Invert the row-major 32x32 block M (ld 33) in place; T is a 32x33 scratch
+// block.  Wave 0 runs both 16x16 inversions; waves 0 and 1 share the
+// products.  All threads of the workgroup must call (barriers inside).
+__device__ void inv32_blocked(cf (*M)[NB + 1], cf (*T)[NB + 1], bool monitor, uint32_t *status, float *ratio_slot) {
+    const int wave = threadIdx.x >> 6;
+    cf *A = &M[0][0], *B = &M[0][16], *C = &M[16][0], *D = &M[16][16];
+    cf *E = &T[0][0], *F = &T[0][16];
+    const int ld = NB + 1;
+    if (wave == 0) inv16_reg(A, ld, monitor, status, ratio_slot);   // Ai
+    lds_barrier();
+    if (wave == 0) {
+        gemm16_wave(C, A, nullptr, E, ld, 1.f);          // E = C Ai
+        gemm16_wave(E, B, D, D, ld, -1.f);               // S = D - E B
+        inv16_reg(D, ld, monitor, status, ratio_slot);   // Si
+    } else if (wave == 1) {
+        gemm16_wave(A, B, nullptr, F, ld, 1.f);          // F = Ai B
+    }
+    lds_barrier();
+    if (wave == 0) gemm16_wave(D, E, nullptr, C, ld, -1.f);        // X21 = -Si E
+    else if (wave == 1) gemm16_wave(F, D, nullptr, B, ld, -1.f);   // X12 = -F Si
+    lds_barrier();
+    if (wave == 0) gemm16_wave(F, C, A, A, ld, -1.f);              // X11 = Ai - F X21
+    lds_barrier();
 }
 
 // D^T quadrant (16 x 16) of D = A B over a 32-deep reduction, computed as
@@ -218,7 +313,15 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(GjParams p) {
         const int row = ti * NB + qr * 16 + li, col = tj * NB + qc * 16 + 4 * kq + r;
         aij[r] = general ? src[(int64_t)row + ld * col] : mk(0.f, 0.f);
     }
-    if (!tk.pin) gj_invert32(a, sh, rowK && colK, p.status, p.pivot_ratio + task);
+    if (!tk.pin) {
+        // first panel: every WG inverts the diagonal block itself
+#pragma unroll
+        for (int e = 0; e < 4; ++e) sh.Pd[i][jb + 8 * e] = a[e];
+        lds_barrier();
+        inv32_blocked(sh.Pd, sh.PT, rowK && colK, p.status, p.pivot_ratio + task);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) a[e] = sh.Pd[i][jb + 8 * e];
+    }
 
     // stage P and the strips for the MFMA products
     lds_barrier();
@@ -270,19 +373,16 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(GjParams p) {
     }
     // the next panel's diagonal tile: invert it here and publish the inverse
     if (tk.pout && ti == q + 1 && tj == q + 1) {
-        // the serial pivot chain is the launch's critical path: let its waves win
+        // the serial inversion is the launch's critical path: let its waves win
         // issue arbitration against the other WGs sharing this CU
         __builtin_amdgcn_s_setprio(3);
         lds_barrier();
 #pragma unroll
         for (int r = 0; r < 4; ++r) sh.Pd[qr * 16 + li][qc * 16 + 4 * kq + r] = out[r];
         lds_barrier();
-        cf nx[4];
+        inv32_blocked(sh.Pd, sh.PT, true, p.status, p.pivot_ratio + task);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) nx[e] = sh.Pd[i][jb + 8 * e];
-        gj_invert32(nx, sh, true, p.status, p.pivot_ratio + task);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) tk.pout[i + NB * (jb + 8 * e)] = nx[e];
+        for (int e = 0; e < 4; ++e) tk.pout[i + NB * (jb + 8 * e)] = sh.Pd[i][jb + 8 * e];
         __builtin_amdgcn_s_setprio(0);
     }
 }

@@ -14,10 +14,10 @@ namespace wust {
 // ---------------------------------------------------------------------------
 // complex arithmetic (interleaved re/im; float2 / double2 storage)
 // ---------------------------------------------------------------------------
-struct cf {
+struct __attribute__((aligned(8))) cf {
     float re, im;
 };
-struct cd {
+struct __attribute__((aligned(16))) cd {
     double re, im;
 };
 
