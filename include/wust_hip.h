@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WUST_ABI_VERSION 1
+#define WUST_ABI_VERSION 2
 
 typedef enum {
     WUST_OK = 0,
@@ -57,6 +57,14 @@ typedef enum {
     WUST_MODE_PLAIN = 2
 } wust_mode;
 
+/* Flag OR-ed into the mode of wust_helmholtz_create (engine modes only):
+ * invert every Schur block by Gauss-Jordan elimination.  Without it, blocks
+ * that have an already-inverted neighbour in their chain are inverted by
+ * Newton-Schulz iteration started from that neighbour's inverse, with a
+ * device-side residual bound; if any block misses the bound the whole
+ * factor is redone by Gauss-Jordan (wust_info.factor_fallback). */
+#define WUST_FACTOR_GAUSS_JORDAN 0x100
+
 typedef struct wust_helmholtz wust_helmholtz;
 
 /* Grid, frequency and PML description: the scalar arguments of
@@ -83,6 +91,9 @@ typedef struct {
     double assembly_ms, factor_ms; /* device time, hipEvent-timed */
     double min_pivot_ratio;        /* smallest |pivot| / max |column entry| seen in the factor */
     int64_t factor_bytes;          /* device bytes held by the factors */
+    double inverse_residual;       /* largest bound on ||I - T_j X_j||_F over Newton-Schulz blocks */
+    int32_t iterative_blocks;      /* Schur blocks inverted by Newton-Schulz */
+    int32_t factor_fallback;       /* 1: Newton-Schulz missed its bound, factor redone by Gauss-Jordan */
 } wust_info;
 
 typedef struct {
@@ -177,8 +188,9 @@ int wust_oracle_release(wust_helmholtz *h);
 /* ---- tooling: per-kernel-class HIP-event timing (no reference counterpart) ------------ */
 /* Classes: 0 assembly, 1 Schur update, 2 Gauss-Jordan panel, 3 sweep operand,
  * 4 sweep GEMM, 5 FP64 residual, 6 source fit, 7 gradient, 8 plain factor step,
- * 9 plain solve step.  When a class bit is set, every launch of that class is
- * bracketed by hipEvents on its stream; wust_profile_read resolves them. */
+ * 9 plain solve step, 10 Newton-Schulz block-inverse GEMM.  When a class bit is
+ * set, every launch of that class is timestamped by its dispatch packet
+ * (hipExtLaunchKernelGGL events); wust_profile_read resolves them. */
 int wust_profile_enable(uint32_t class_mask);
 int wust_profile_read(int kernel_class, int64_t *count, double *total_ms);
 int wust_profile_reset(void);

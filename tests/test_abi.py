@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), f"{name} missing from {N.library_path()}"
         assert name in N.EXPORTS, f"{name} has no ctypes prototype"
-    assert lib.wust_abi_version() == 1
+    assert lib.wust_abi_version() == 2
 
 
 def test_status_ids_match_reference_identifiers():
@@ -67,6 +67,9 @@ def test_argument_validation_without_device():
     assert b"stencil_bounds" in lib.wust_last_error()
     desc.stencil_vmin = 1400
     assert lib.wust_helmholtz_create(C.byref(desc), fake, None, 7, None, C.byref(h)) == 1
+    assert b"unknown mode" in lib.wust_last_error()
+    assert lib.wust_helmholtz_create(C.byref(desc), fake, None, 0x200, None, C.byref(h)) == 1
+    assert b"unknown mode flag" in lib.wust_last_error()
     assert not h.value
 
 
@@ -75,7 +78,9 @@ def test_python_mirror_rejects_like_the_reference():
     x = np.arange(31) * 1e-3
     c = 1500 * np.ones((31, 31))
     for bad in [dict(stencil_bounds=[1400, 1700]), dict(backend="gpu"), dict(backend="gpu", stencil_bounds=None),
-                dict(backend="gpu", stencil_bounds=[1500]), dict(backend="tpu", stencil_bounds=[1400, 1700])]:
+                dict(backend="gpu", stencil_bounds=[1500]), dict(backend="tpu", stencil_bounds=[1400, 1700]),
+                dict(backend="gpu", stencil_bounds=[1400, 1700], factor="lu"),
+                dict(backend="gpu", stencil_bounds=[1400, 1700], mode="plain", factor="gauss-jordan")]:
         with pytest.raises(WustError) as ei:
             HelmholtzSolver(x, x, c, np.zeros_like(c), 1e5, -1, 10, .004, bad)
         assert ei.value.identifier == "WUST:InvalidInput"

@@ -1,0 +1,78 @@
+"""GPU checks of the two ways the engine inverts Schur blocks (needs an MI355X).
+
+Blocks with an inverted neighbour in their chain are inverted by Newton-Schulz
+iteration (factor='auto'); the rest, and the whole factor when an iteration
+misses its residual bound, by Gauss-Jordan elimination (factor='gauss-jordan'
+forces it everywhere).  Tolerances:
+* the two methods give complex-FP32 solutions that agree to <= 1e-5
+  relative (both are FP32 inverses with residuals near 1e-6);
+* the Newton-Schulz residual bound reported by the factor is <= 1e-5;
+* refined solves from either factor agree with each other to <= 1e-12;
+* a forced fallback reproduces the Gauss-Jordan factor bit for bit.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from waveforminversionust_amd import workloads as W  # noqa: E402
+from waveforminversionust_amd import helmholtz as G  # noqa: E402
+
+OPTS = dict(backend="gpu", stencil_bounds=[1400, 1700])
+
+
+@pytest.fixture(scope="module")
+def ring():
+    return W.cpu_reference_case()
+
+
+def _solver(case, mode, factor):
+    return G.HelmholtzSolver(case.x_m, case.y_m, case.c_true, None, case.frequencies_hz[0], -1,
+                             case.cfg["pml_strength"], case.cfg["pml_m"], dict(OPTS, mode=mode, factor=factor))
+
+
+def _sources(case, count):
+    src = torch.zeros((case.ny * case.nx, count), dtype=torch.complex128, device="cuda")
+    src[torch.as_tensor(case.tx_index[:count] - 1), torch.arange(count)] = 1
+    return src.reshape(case.nx, case.ny, count).permute(1, 0, 2)
+
+
+def _rel(a, b):
+    return float((a - b).abs().max() / b.abs().max())
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_newton_schulz_matches_gauss_jordan(ring, adjoint):
+    src = _sources(ring, 16)
+    s_ns = _solver(ring, "c64", "auto")
+    s_gj = _solver(ring, "c64", "gauss-jordan")
+    i_ns, i_gj = s_ns.info(), s_gj.info()
+    m = ring.nx - 2
+    assert i_ns["iterative_blocks"] == m - 3 and i_ns["factor_fallback"] == 0
+    assert 0 < i_ns["inverse_residual"] <= 1e-5
+    assert i_gj["iterative_blocks"] == 0 and i_gj["factor_fallback"] == 0
+    u_ns, _ = s_ns.solve(src, adjoint)
+    u_gj, _ = s_gj.solve(src, adjoint)
+    assert _rel(u_ns, u_gj) <= 1e-5
+
+
+def test_newton_schulz_refined_agrees(ring):
+    src = _sources(ring, 8)
+    u_ns, _ = _solver(ring, "refined", "auto").solve(src, False, tol=1e-13)
+    u_gj, _ = _solver(ring, "refined", "gauss-jordan").solve(src, False, tol=1e-13)
+    assert _rel(u_ns, u_gj) <= 1e-12
+
+
+def test_newton_schulz_fallback_is_gauss_jordan(ring, monkeypatch):
+    # one update per block cannot reach the bound in the PML columns
+    monkeypatch.setenv("WUST_NS_MAX_UPDATES", "1")
+    s_fb = _solver(ring, "c64", "auto")
+    info = s_fb.info()
+    assert info["factor_fallback"] == 1 and info["iterative_blocks"] == 0
+    monkeypatch.delenv("WUST_NS_MAX_UPDATES")
+    s_gj = _solver(ring, "c64", "gauss-jordan")
+    src = _sources(ring, 8)
+    u_fb, _ = s_fb.solve(src, False)
+    u_gj, _ = s_gj.solve(src, False)
+    assert torch.equal(u_fb, u_gj)

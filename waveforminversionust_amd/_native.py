@@ -19,6 +19,8 @@ STATUS_IDS = {
 
 MODE_REFINED, MODE_C64, MODE_PLAIN = 0, 1, 2
 MODES = {"refined": MODE_REFINED, "c64": MODE_C64, "plain": MODE_PLAIN}
+FACTOR_GAUSS_JORDAN = 0x100
+FACTORS = {"auto": 0, "gauss-jordan": FACTOR_GAUSS_JORDAN}
 
 
 class WustError(RuntimeError):
@@ -48,6 +50,7 @@ class Info(C.Structure):
         ("b", C.c_double), ("d", C.c_double), ("e", C.c_double),
         ("assembly_ms", C.c_double), ("factor_ms", C.c_double),
         ("min_pivot_ratio", C.c_double), ("factor_bytes", C.c_int64),
+        ("inverse_residual", C.c_double), ("iterative_blocks", C.c_int32), ("factor_fallback", C.c_int32),
     ]
 
 
@@ -86,7 +89,7 @@ EXPORTS = {
 }
 
 PROFILE_CLASSES = ["assembly", "schur", "gj_panel", "sweep_operand", "sweep_gemm", "residual", "source_fit",
-                   "gradient", "plain_factor_step", "plain_solve_step"]
+                   "gradient", "plain_factor_step", "plain_solve_step", "ns_gemm"]
 
 _lib = None
 

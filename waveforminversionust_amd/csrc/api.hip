@@ -141,6 +141,8 @@ struct wust_helmholtz {
         dfree(op.pages);
         dfree(op.work);
         dfree(op.pbuf);
+        dfree(op.nswork);
+        dfree(op.ns_norm);
         dfree(op.status);
         dfree(idx);
     }
@@ -348,7 +350,10 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
     if (out) *out = nullptr;
     return guarded([&] {
         require(desc && d_vel && out, "null argument");
+        const int flags = mode & ~0xff;
+        mode &= 0xff;
         require(mode == WUST_MODE_REFINED || mode == WUST_MODE_C64 || mode == WUST_MODE_PLAIN, "unknown mode");
+        require((flags & ~WUST_FACTOR_GAUSS_JORDAN) == 0, "unknown mode flag");
         const wust_grid_desc &g = *desc;
         require(g.ny >= 3 && g.nx >= 3, "Expected a positive finite [ny,nx] medium (min size 3)");
         require(std::isfinite(g.dx) && std::isfinite(g.dy) && g.dx > 0 && g.dy > 0, "Uniform increasing grids required");
@@ -373,6 +378,7 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         op.npad = (int)round_up(op.n, 32);
         op.p = op.m / 2;
         op.mode = mode;
+        op.factor_method = (flags & WUST_FACTOR_GAUSS_JORDAN) ? FACTOR_GJ : FACTOR_AUTO;
         stencil_params(g.stencil_vmin, g.stencil_vmax, g.frequency_hz, g.dx, g.dy / g.dx, op.b, op.d, op.e);
         AssemblyParams &ap = op.ap;
         ap.ny = g.ny;
@@ -397,13 +403,17 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         op.vsm = dalloc<cd>(nodes);
         op.w64 = dalloc<cd>(9 * (int64_t)op.m * op.n);
         op.w32 = dalloc<cf>(9 * (int64_t)op.m * op.npad);
-        op.status = dalloc<uint32_t>(4);
+        op.status = dalloc<uint32_t>(8);
         const int64_t pg = (int64_t)op.npad * op.npad;
         op.pages = dalloc<cf>(pg * op.m);
         op.work = dalloc<cf>(4 * pg);
         op.pbuf = dalloc<cf>(4 * 32 * 32);
+        if (mode != WUST_MODE_PLAIN && op.factor_method == FACTOR_AUTO) {
+            op.nswork = dalloc<cf>(4 * pg);
+            op.ns_norm = dalloc<float>(2 * 8);
+        }
         op.factor_bytes = (int64_t)sizeof(cf) * pg * op.m;
-        WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, 4 * sizeof(uint32_t), s));
+        WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, 8 * sizeof(uint32_t), s));
 
         hipEvent_t e0, e1, e2;
         WUST_HIP_CHECK(hipEventCreate(&e0));
@@ -457,6 +467,9 @@ int wust_helmholtz_info(const wust_helmholtz *h, wust_info *info) {
         info->factor_ms = op.factor_ms;
         info->min_pivot_ratio = op.min_pivot_ratio;
         info->factor_bytes = op.factor_bytes;
+        info->inverse_residual = op.inverse_residual;
+        info->iterative_blocks = op.ns_blocks;
+        info->factor_fallback = op.factor_fallback;
     });
 }
 

@@ -13,9 +13,15 @@ namespace wust {
 //   pages [m][npad][npad]  inverse Schur blocks X_j (twisted: left chain j < p, right chain
 //                          j > p, middle j = p); padding rows/cols hold the identity
 //   solve fields  [m][npad][Rpad]   interior block x, row y, RHS r fastest
+// How Schur blocks are inverted (FACTOR_AUTO: Newton-Schulz from the
+// neighbouring inverse where a chain has one, Gauss-Jordan elsewhere and as
+// the fallback; FACTOR_GJ: Gauss-Jordan for every block).
+enum FactorMethod { FACTOR_AUTO = 0, FACTOR_GJ = 1 };
+
 struct Operator {
     int ny = 0, nx = 0, n = 0, m = 0, npad = 0, p = 0;  // p: twisted meeting block
     int mode = 0;
+    int factor_method = FACTOR_AUTO;
     double b = 0, d = 0, e = 0;
     AssemblyParams ap{};
     cd *ck2 = nullptr, *vsm = nullptr, *w64 = nullptr;
@@ -23,8 +29,14 @@ struct Operator {
     cf *pages = nullptr;
     cf *work = nullptr;           // factor ping-pong buffers: [2 chains][2][npad*npad]
     cf *pbuf = nullptr;           // diagonal-block inverses handed between panel launches: [2 chains][2][32*32]
-    uint32_t *status = nullptr;   // device status words: [0] flags, [2..3] pivot ratios (float)
+    cf *nswork = nullptr;         // Newton-Schulz iterates: [2 chains][2][npad*npad]
+    float *ns_norm = nullptr;     // ||E_k||_F^2 per chain and update: [2 chains][NS_KMAX = 8]
+    uint32_t *status = nullptr;   // device status words: [0] flags, [2..3] pivot ratios (float),
+                                  // [4] largest Newton-Schulz residual bound (float)
     double assembly_ms = 0, factor_ms = 0, min_pivot_ratio = 1.0;
+    double inverse_residual = 0;  // max over Newton-Schulz blocks of the bound on ||I - T_j X_j||_F
+    int ns_blocks = 0;            // blocks inverted by Newton-Schulz
+    int factor_fallback = 0;      // 1: Newton-Schulz missed its bound and every block was redone by Gauss-Jordan
     int64_t factor_bytes = 0;
 
     const cf *coef(int dir, int block) const { return w32 + ((int64_t)dir * m + block) * npad; }

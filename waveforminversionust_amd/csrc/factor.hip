@@ -22,6 +22,9 @@
 //     WUST_SINGULAR_SCHUR_BLOCK, and the smallest |pivot| / max|entry below
 //     it| inside each 16x16 pivot block is tracked so that a loss of
 //     dominance is visible.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <utility>
 
 #include "engine.h"
@@ -38,11 +41,14 @@ struct SchurTerm {
     const cf *Pn, *Pc, *Ps;      // left tridiagonal factor, row coefficients
     const cf *Qn, *Qc, *Qs;      // right tridiagonal factor, row coefficients
 };
+constexpr int NS_KMAX = 8;  // Newton-Schulz updates per block, upper limit
+
 struct SchurTask {
     const cf *Dn, *Dc, *Ds;
     SchurTerm t[2];
     int nterms;
     cf *T;
+    float *clear_norms;   // non-null: NS_KMAX Newton-Schulz norm slots zeroed for this block
 };
 struct SchurParams {
     SchurTask task[2];
@@ -54,6 +60,7 @@ __global__ void __launch_bounds__(256) schur_kernel(SchurParams p) {
     const SchurTask &tk = p.task[blockIdx.z];
     int r = blockIdx.x * 256 + threadIdx.x;
     int c = blockIdx.y;
+    if (tk.clear_norms && blockIdx.x == 0 && c == 0 && threadIdx.x < NS_KMAX) tk.clear_norms[threadIdx.x] = 0.f;
     if (r >= p.npad) return;
     const int n = p.n, ld = p.npad;
     cf val = mk(0.f, 0.f);
@@ -387,6 +394,187 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(GjParams p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Newton-Schulz block inverse.  Consecutive Schur blocks of one chain differ
+// little away from the PML (T_j X_{j-1} is close to I), so X_j is reached from
+// X_0 = X_{j-1} by
+//   E_k = I - T X_k,   X_{k+1} = X_k + X_k E_k      (E_{k+1} = E_k^2)
+// on MFMA instead of a serial pivot chain.  ns_gemm_kernel computes one such
+// product for up to two chains:  out = C0 + sgn * A B  (C0 = I when null), all
+// npad x npad column-major, optionally accumulating ||out||_F^2.  The
+// iteration count is decided on the device: an update is the last one when
+// ||E_k||_F^2 <= thr2 (the residual after it is bounded by ||E_k||_F^2) or when
+// the host-side iteration budget is spent; launches after the last update of
+// a chain return at once.
+// ---------------------------------------------------------------------------
+struct NsTask {
+    const cf *A, *B;
+    const cf *C0;             // null: identity
+    cf *out_next;             // output when further updates follow
+    cf *out_final;            // output of the last update (null: out_next)
+    float *norm_out;          // non-null: accumulates ||out||_F^2
+    const float *done_norm;   // non-null: the chain is finished when *done_norm <= thr2
+    const float *cur_norm;    // non-null: this update is the last when *cur_norm <= thr2
+    int force_final;          // this update is the last regardless
+    float sgn;
+};
+struct NsParams {
+    NsTask task[2];
+    int ntasks, npad;
+    float thr2;        // ||E_k||_F^2 below which one more update reaches the FP32 floor
+    float accept;      // largest acceptable bound on ||I - T X||_F after the last update
+    uint32_t *status;
+    float *resid_max;  // running max of that bound over the factor (float bits, atomicMax)
+};
+
+// Tile order inside one task: the four quadrants of the T x T tile grid one
+// after another, so that a contiguous run of tiles (one XCD's share) touches
+// half of the rows of A and half of the columns of B.
+__device__ __forceinline__ void ns_tile(int r, int T, int &ti, int &tj) {
+    const int h0 = (T + 1) / 2, h1 = T / 2;
+    const int rows[4] = {h0, h0, h1, h1}, cols[4] = {h0, h1, h0, h1};
+    const int r0[4] = {0, 0, h0, h0}, c0[4] = {0, h0, 0, h0};
+    int q = 0;
+    while (q < 3 && r >= rows[q] * cols[q]) {
+        r -= rows[q] * cols[q];
+        ++q;
+    }
+    ti = r0[q] + r / cols[q];
+    tj = c0[q] + r % cols[q];
+}
+
+// One 32x32 output tile per workgroup; the four waves split the reduction
+// (k-blocks of 16 dealt round-robin) and each computes the whole tile with
+// 2x2 MFMA 16x16x4 blocks.  Inside a k-block, lane (g = lane / 16, c = lane % 16)
+// contracts k = k0 + 4g + s at MFMA step s, so its B operand for the four
+// steps is four consecutive complex values of one column (two 16-byte
+// loads) and the four lanes of a column read one 128-byte line.
+__global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
+    const int T = p.npad / NB, per_task = T * T, G = p.ntasks * per_task;
+    const int per_xcd = (G + 7) / 8;
+    const int L = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (L >= G) return;
+    const int task = L / per_task;
+    const NsTask &tk = p.task[task];
+    if (tk.done_norm && *tk.done_norm <= p.thr2) return;
+    const bool last = tk.force_final || (tk.cur_norm && *tk.cur_norm <= p.thr2);
+    cf *out = (last && tk.out_final) ? tk.out_final : tk.out_next;
+    int ti, tj;
+    ns_tile(L % per_task, T, ti, tj);
+
+    __shared__ float4 red[4][4][2][64];   // [wave][block][half][lane]
+    __shared__ float wsum[4];
+    const int ld = p.npad, i0 = ti * NB, j0 = tj * NB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const cf *A = tk.A, *B = tk.B;
+
+    v4f dr[2][2], di[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) dr[x][y] = di[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    const int KB = ld / 16;
+    cf a[2][4], bm[2][4], an[2][4], bn[2][4];
+    auto load = [&](int kb, cf (&av)[2][4], cf (&bv)[2][4]) {
+        const int k0 = kb * 16 + 4 * g;
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) av[rb][s] = A[(int64_t)(k0 + s) * ld + i0 + 16 * rb + c];
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+            const float4 *src = reinterpret_cast<const float4 *>(B + (int64_t)(j0 + 16 * cb + c) * ld + k0);
+            const float4 u0 = src[0], u1 = src[1];
+            bv[cb][0] = mk(u0.x, u0.y);
+            bv[cb][1] = mk(u0.z, u0.w);
+            bv[cb][2] = mk(u1.x, u1.y);
+            bv[cb][3] = mk(u1.z, u1.w);
+        }
+    };
+    int kb = wave;
+    if (kb < KB) load(kb, a, bm);
+    while (kb < KB) {
+        const int kn = kb + 4;
+        if (kn < KB) load(kn, an, bn);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+                    const cf x = a[rb][s], y = bm[cb][s];
+                    dr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.re, dr[rb][cb], 0, 0, 0);
+                    dr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-x.im, y.im, dr[rb][cb], 0, 0, 0);
+                    di[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.im, di[rb][cb], 0, 0, 0);
+                    di[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.im, y.re, di[rb][cb], 0, 0, 0);
+                }
+        if (kn < KB) {
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    a[x][s] = an[x][s];
+                    bm[x][s] = bn[x][s];
+                }
+        }
+        kb = kn;
+    }
+
+    // reduce the four k-partials: wave w finishes block (w / 2, w % 2)
+#pragma unroll
+    for (int blk = 0; blk < 4; ++blk) {
+        const v4f &r = dr[blk >> 1][blk & 1], &im = di[blk >> 1][blk & 1];
+        red[wave][blk][0][lane] = make_float4(r[0], r[1], r[2], r[3]);
+        red[wave][blk][1][lane] = make_float4(im[0], im[1], im[2], im[3]);
+    }
+    lds_barrier();
+    float sr[4] = {0.f, 0.f, 0.f, 0.f}, si[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const float4 u = red[w][wave][0][lane], v = red[w][wave][1][lane];
+        sr[0] += u.x; sr[1] += u.y; sr[2] += u.z; sr[3] += u.w;
+        si[0] += v.x; si[1] += v.y; si[2] += v.z; si[3] += v.w;
+    }
+    // lane holds rows i0 + 16 rb + 4 g + r (r = 0..3) of column j0 + 16 cb + c
+    const int rb = wave >> 1, cb = wave & 1;
+    const int row = i0 + 16 * rb + 4 * g, col = j0 + 16 * cb + c;
+    const int64_t off = (int64_t)col * ld + row;
+    cf v[4];
+    if (tk.C0) {
+        const float4 *src = reinterpret_cast<const float4 *>(tk.C0 + off);
+        const float4 u0 = src[0], u1 = src[1];
+        v[0] = mk(u0.x, u0.y);
+        v[1] = mk(u0.z, u0.w);
+        v[2] = mk(u1.x, u1.y);
+        v[3] = mk(u1.z, u1.w);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = mk(row + r == col ? 1.f : 0.f, 0.f);
+    }
+    float n2 = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        v[r] = mk(v[r].re + tk.sgn * sr[r], v[r].im + tk.sgn * si[r]);
+        n2 += abs2(v[r]);
+    }
+    float4 *dst = reinterpret_cast<float4 *>(out + off);
+    dst[0] = make_float4(v[0].re, v[0].im, v[1].re, v[1].im);
+    dst[1] = make_float4(v[2].re, v[2].im, v[3].re, v[3].im);
+
+    if (tk.norm_out) {
+        for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o, 64);
+        if (lane == 0) wsum[wave] = n2;
+        lds_barrier();
+        if (threadIdx.x == 0) atomicAdd(tk.norm_out, wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+    }
+    if (last && tk.cur_norm && L % per_task == 0 && threadIdx.x == 0) {
+        const float bound = *tk.cur_norm;   // ||I - T X_{k+1}||_F <= ||E_k||_F^2
+        if (!(bound <= p.accept)) atomicOr(p.status, (uint32_t)ST_INVERSE_UNCONVERGED);
+        if (bound >= 0.f) atomicMax(reinterpret_cast<int *>(p.resid_max), __float_as_int(bound));
+    }
+}
+
 struct ChainStep {
     int block;      // block index j to produce
     int kind;       // 0: left chain, 1: right chain, 2: middle
@@ -423,15 +611,41 @@ void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s) {
     launch_k(PC_SCHUR, schur_kernel, grid, dim3(256), 0, s, sp);
 }
 
-void factor_engine(Operator &op, hipStream_t s) {
-    const int m = op.m, n = op.n, ld = op.npad, p = op.p;
-    const int64_t pg = (int64_t)ld * ld;
-    int Ldirs[3] = {DNW, DW, DSW};
-    int Udirs[3] = {DNE, DE, DSE};
-    float *ratio_d = reinterpret_cast<float *>(op.status + 2);
-    float one[2] = {1.0f, 1.0f};
-    WUST_HIP_CHECK(hipMemcpyAsync(ratio_d, one, sizeof(one), hipMemcpyHostToDevice, s));
-    auto run_steps = [&](const std::vector<ChainStep> &steps) {
+namespace {
+
+// Newton-Schulz controls.  An update is the last one once ||E_k||_F <= 1e-3
+// (the residual after it is <= 1e-6, the FP32 floor of these blocks); a
+// block whose bound after its last update exceeds 1e-4 sends the whole
+// factor back to Gauss-Jordan.
+constexpr float NS_THR2 = 1e-6f;
+constexpr float NS_ACCEPT = 1e-4f;
+
+struct FactorRun {
+    Operator &op;
+    hipStream_t s;
+    bool allow_ns;
+    int pml_cells;
+    int ns_blocks = 0;
+    int update_cap = NS_KMAX;
+
+    // Newton-Schulz update budget of chain step `st`: the PML columns at
+    // the start of each chain change fastest.
+    int ns_budget(int st) const {
+        const int b = st < pml_cells + 8 ? NS_KMAX : 4;
+        return b < update_cap ? b : update_cap;
+    }
+
+    void run_steps(const std::vector<ChainStep> &steps, int st) {
+        const int m = op.m, n = op.n, ld = op.npad;
+        const int64_t pg = (int64_t)ld * ld;
+        int Ldirs[3] = {DNW, DW, DSW};
+        int Udirs[3] = {DNE, DE, DSE};
+        float *ratio_d = reinterpret_cast<float *>(op.status + 2);
+        std::vector<int> gj, ns;
+        for (int c = 0; c < (int)steps.size(); ++c) {
+            const bool use_ns = allow_ns && op.nswork && steps[c].kind != 2 && st > 0;
+            (use_ns ? ns : gj).push_back(c);
+        }
         SchurParams sp{};
         sp.ntasks = (int)steps.size();
         sp.n = n;
@@ -444,6 +658,7 @@ void factor_engine(Operator &op, hipStream_t s) {
             tk.Ds = op.coef(DS, j);
             tk.nterms = 0;
             tk.T = op.work + (int64_t)c * 2 * pg;
+            tk.clear_norms = (allow_ns && op.ns_norm) ? op.ns_norm + c * NS_KMAX : nullptr;
             int kind = steps[c].kind;
             if ((kind == 0 || kind == 2) && j >= 1)  // - L_j X^L_{j-1} U_{j-1}
                 add_term(op, tk.t[tk.nterms++], op.page(j - 1), j, Ldirs, j - 1, Udirs);
@@ -452,42 +667,124 @@ void factor_engine(Operator &op, hipStream_t s) {
         }
         dim3 grid((ld + 255) / 256, ld, sp.ntasks);
         launch_k(PC_SCHUR, schur_kernel, grid, dim3(256), 0, s, sp);
-        const int Q = ld / NB;
-        for (int q = 0; q < Q; ++q) {
-            GjParams gp{};
-            gp.ntasks = sp.ntasks;
-            gp.npad = ld;
-            gp.q = q;
-            gp.status = op.status;
-            gp.pivot_ratio = ratio_d;
-            for (int c = 0; c < sp.ntasks; ++c) {
-                cf *b0 = op.work + (int64_t)c * 2 * pg;
-                cf *b1 = b0 + pg;
-                gp.task[c].src = (q % 2 == 0) ? b0 : b1;
-                gp.task[c].dst = (q == Q - 1) ? op.page(steps[c].block) : ((q % 2 == 0) ? b1 : b0);
-                cf *pb = op.pbuf + (int64_t)c * 2 * NB * NB;
-                gp.task[c].pin = (q == 0) ? nullptr : pb + (q % 2) * NB * NB;
-                gp.task[c].pout = (q + 1 < Q) ? pb + ((q + 1) % 2) * NB * NB : nullptr;
+
+        if (!gj.empty()) {
+            const int Q = ld / NB;
+            for (int q = 0; q < Q; ++q) {
+                GjParams gp{};
+                gp.ntasks = (int)gj.size();
+                gp.npad = ld;
+                gp.q = q;
+                gp.status = op.status;
+                gp.pivot_ratio = ratio_d;
+                for (int t = 0; t < gp.ntasks; ++t) {
+                    const int c = gj[t];
+                    cf *b0 = op.work + (int64_t)c * 2 * pg;
+                    cf *b1 = b0 + pg;
+                    gp.task[t].src = (q % 2 == 0) ? b0 : b1;
+                    gp.task[t].dst = (q == Q - 1) ? op.page(steps[c].block) : ((q % 2 == 0) ? b1 : b0);
+                    cf *pb = op.pbuf + (int64_t)c * 2 * NB * NB;
+                    gp.task[t].pin = (q == 0) ? nullptr : pb + (q % 2) * NB * NB;
+                    gp.task[t].pout = (q + 1 < Q) ? pb + ((q + 1) % 2) * NB * NB : nullptr;
+                }
+                launch_k(PC_GJ_PANEL, gj_panel_kernel, dim3(gp.ntasks * Q * Q), dim3(256), 0, s, gp);
             }
-            launch_k(PC_GJ_PANEL, gj_panel_kernel, dim3(sp.ntasks * (ld / NB) * (ld / NB)), dim3(256), 0, s, gp);
         }
-    };
-
-    // chains: left j = 0..p-1, right j = m-1..p+1, then the middle block p
-    int nl = p, nr = m - 1 - p;
-    int nsteps = nl > nr ? nl : nr;
-    for (int st = 0; st < nsteps; ++st) {
-        std::vector<ChainStep> steps;
-        if (st < nl) steps.push_back({st, 0});
-        if (st < nr) steps.push_back({m - 1 - st, 1});
-        run_steps(steps);
+        if (!ns.empty()) {
+            const int K = ns_budget(st);
+            const int T = ld / NB;
+            const int G = (int)ns.size() * T * T;
+            const int grid_ns = 8 * ((G + 7) / 8);
+            for (int k = 0; k < K; ++k) {
+                NsParams pe{}, px{};
+                pe.ntasks = px.ntasks = (int)ns.size();
+                pe.npad = px.npad = ld;
+                pe.thr2 = px.thr2 = NS_THR2;
+                pe.accept = px.accept = NS_ACCEPT;
+                pe.status = px.status = op.status;
+                pe.resid_max = px.resid_max = reinterpret_cast<float *>(op.status + 4);
+                for (int t = 0; t < (int)ns.size(); ++t) {
+                    const int c = ns[t];
+                    const int j = steps[c].block;
+                    const cf *T0 = op.work + (int64_t)c * 2 * pg;     // Schur block
+                    cf *E = op.work + (int64_t)c * 2 * pg + pg;
+                    cf *xb = op.nswork + (int64_t)c * 2 * pg;
+                    const cf *Xk = (k == 0) ? op.page(steps[c].kind == 0 ? j - 1 : j + 1) : xb + ((k - 1) % 2) * pg;
+                    float *nrm = op.ns_norm + c * NS_KMAX;
+                    NsTask &te = pe.task[t];
+                    te.A = T0;
+                    te.B = Xk;
+                    te.C0 = nullptr;
+                    te.sgn = -1.f;
+                    te.out_next = E;
+                    te.norm_out = nrm + k;
+                    te.done_norm = (k > 0) ? nrm + k - 1 : nullptr;
+                    NsTask &tx = px.task[t];
+                    tx.A = Xk;
+                    tx.B = E;
+                    tx.C0 = Xk;
+                    tx.sgn = 1.f;
+                    tx.out_next = xb + (k % 2) * pg;
+                    tx.out_final = op.page(j);
+                    tx.done_norm = (k > 0) ? nrm + k - 1 : nullptr;
+                    tx.cur_norm = nrm + k;
+                    tx.force_final = (k == K - 1);
+                }
+                launch_k(PC_NS_GEMM, ns_gemm_kernel, dim3(grid_ns), dim3(256), 0, s, pe);
+                launch_k(PC_NS_GEMM, ns_gemm_kernel, dim3(grid_ns), dim3(256), 0, s, px);
+            }
+            ns_blocks += (int)ns.size();
+        }
     }
-    run_steps({{p, 2}});
 
-    float r[2];
-    WUST_HIP_CHECK(hipMemcpyAsync(r, ratio_d, sizeof(r), hipMemcpyDeviceToHost, s));
+    void run() {
+        const int m = op.m, p = op.p;
+        // chains: left j = 0..p-1, right j = m-1..p+1, then the middle block p
+        int nl = p, nr = m - 1 - p;
+        int nsteps = nl > nr ? nl : nr;
+        for (int st = 0; st < nsteps; ++st) {
+            std::vector<ChainStep> steps;
+            if (st < nl) steps.push_back({st, 0});
+            if (st < nr) steps.push_back({m - 1 - st, 1});
+            run_steps(steps, st);
+        }
+        run_steps({{p, 2}}, 0);
+    }
+};
+
+}  // namespace
+
+void factor_engine(Operator &op, hipStream_t s) {
+    const uint32_t init[6] = {0u, 0u, 0x3f800000u, 0x3f800000u, 0u, 0u};   // flags, -, ratios 1.0f, residual 0
+    WUST_HIP_CHECK(hipMemcpyAsync(op.status, init, sizeof(init), hipMemcpyHostToDevice, s));
+    const int pml_cells = (int)std::ceil(op.ap.L / op.ap.dx);
+    FactorRun run{op, s, op.factor_method == FACTOR_AUTO, pml_cells};
+    // WUST_NS_MAX_UPDATES caps the Newton-Schulz updates per block (tuning;
+    // a cap too small for the PML blocks exercises the Gauss-Jordan fallback)
+    if (const char *cap = std::getenv("WUST_NS_MAX_UPDATES")) {
+        const int c = std::atoi(cap);
+        if (c >= 1 && c <= NS_KMAX) run.update_cap = c;
+    }
+    run.run();
+    uint32_t st[6];
+    WUST_HIP_CHECK(hipMemcpyAsync(st, op.status, sizeof(st), hipMemcpyDeviceToHost, s));
     WUST_HIP_CHECK(hipStreamSynchronize(s));
+    op.factor_fallback = 0;
+    if (st[0] & ST_INVERSE_UNCONVERGED) {
+        // some block's Newton-Schulz iteration missed its bound: redo every block by Gauss-Jordan
+        WUST_HIP_CHECK(hipMemcpyAsync(op.status, init, sizeof(init), hipMemcpyHostToDevice, s));
+        FactorRun again{op, s, false, pml_cells};
+        again.run();
+        WUST_HIP_CHECK(hipMemcpyAsync(st, op.status, sizeof(st), hipMemcpyDeviceToHost, s));
+        WUST_HIP_CHECK(hipStreamSynchronize(s));
+        op.factor_fallback = 1;
+        run.ns_blocks = 0;
+    }
+    float r[3];
+    std::memcpy(r, st + 2, sizeof(r));
     op.min_pivot_ratio = r[0] < r[1] ? r[0] : r[1];
+    op.inverse_residual = r[2];
+    op.ns_blocks = run.ns_blocks;
 }
 
 }  // namespace wust

@@ -17,6 +17,7 @@ enum ProfClass {
     PC_GRADIENT,
     PC_PLAIN_FACTOR_STEP,
     PC_PLAIN_SOLVE_STEP,
+    PC_NS_GEMM,
     PC_COUNT
 };
 

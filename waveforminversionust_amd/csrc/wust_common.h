@@ -101,6 +101,7 @@ enum StatusBits : uint32_t {
     ST_NONFINITE_PIVOT = 2u,
     ST_SCALE_FAILURE = 4u,
     ST_NONFINITE = 8u,
+    ST_INVERSE_UNCONVERGED = 16u,   // a Newton-Schulz block inverse missed its residual bound
 };
 
 // ---------------------------------------------------------------------------

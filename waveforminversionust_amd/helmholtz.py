@@ -83,7 +83,10 @@ class HelmholtzSolver:
     ``options`` must hold ``backend`` ('gpu') and ``stencil_bounds`` [min, max];
     optional ``dispersion`` = dict(time_step_s, reference_speed_mps) selects
     kwave-ldr9, ``mode`` in {'refined', 'c64', 'plain'} selects the arithmetic
-    (default 'refined'), and ``tol`` / ``max_iter`` the refinement stop.
+    (default 'refined'), ``factor`` in {'auto', 'gauss-jordan'} how the engine
+    inverts Schur blocks (default 'auto': Newton-Schulz from the neighbouring
+    inverse with a Gauss-Jordan fallback), and ``tol`` / ``max_iter`` the
+    refinement stop.
     """
 
     def __init__(self, x, y, vel, atten, f, signConvention, a0, L_PML, options):
@@ -102,6 +105,12 @@ class HelmholtzSolver:
         if mode not in N.MODES:
             raise WustError("WUST:InvalidInput", f"unknown mode {mode!r}")
         self.mode = mode
+        factor = options.get("factor", "auto")
+        if factor not in N.FACTORS:
+            raise WustError("WUST:InvalidInput", f"unknown factor method {factor!r}")
+        if factor != "auto" and mode == "plain":
+            raise WustError("WUST:InvalidInput", "the plain mode has a single factor method")
+        self.factor = factor
         self.tol = float(options.get("tol", DEFAULT_TOL))
         self.max_iter = int(options.get("max_iter", DEFAULT_MAX_ITER))
         device = _dev(options.get("device"))
@@ -145,7 +154,7 @@ class HelmholtzSolver:
             N.check(self._lib.wust_helmholtz_create(
                 C.byref(desc), C.c_void_p(vel_d.data_ptr()),
                 C.c_void_p(att_d.data_ptr() if att_d is not None else None),
-                N.MODES[mode], _stream(device), C.byref(h)))
+                N.MODES[mode] | N.FACTORS[factor], _stream(device), C.byref(h)))
         self._h = h
         self.wall_seconds = time.perf_counter() - t0
         info = self.info()
@@ -261,7 +270,7 @@ def wust_oracle(c, obs, fi, cfg, nargout=2):
     disp = cfg.get("dispersion") if cfg.get("wavenumber") == "kwave-ldr9" else None
     options = dict(backend=cfg.get("backend", "gpu"), stencil_bounds=cfg["stencil_bounds"], dispersion=disp,
                    mode=cfg.get("mode", "refined"), tol=cfg.get("tol", DEFAULT_TOL),
-                   max_iter=cfg.get("max_iter", DEFAULT_MAX_ITER))
+                   max_iter=cfg.get("max_iter", DEFAULT_MAX_ITER), factor=cfg.get("factor", "auto"))
     if "device" in cfg:
         options["device"] = cfg["device"]
     solver = HelmholtzSolver(obs["x_m"], obs["y_m"], c, None, float(obs["frequencies_hz"][fi]), -1,
