@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--tol", type=float, default=1e-10, help="refinement stop (relative FP64 residual)")
+    ap.add_argument("--tol", type=float, default=3e-10, help="refinement stop (relative FP64 residual)")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 oracle accuracy / CPU baseline leg")

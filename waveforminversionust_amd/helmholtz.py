@@ -28,7 +28,7 @@ import torch
 from . import _native as N
 from ._native import WustError
 
-DEFAULT_TOL = 1e-10
+DEFAULT_TOL = 3e-10
 DEFAULT_MAX_ITER = 30
 
 
