@@ -409,7 +409,7 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         op.work = dalloc<cf>(4 * pg);
         op.pbuf = dalloc<cf>(4 * 32 * 32);
         if (mode != WUST_MODE_PLAIN && op.factor_method == FACTOR_AUTO) {
-            op.nswork = dalloc<cf>(4 * pg);
+            op.nswork = dalloc<cf>(6 * pg);
             op.ns_norm = dalloc<float>(2 * 8);
         }
         op.factor_bytes = (int64_t)sizeof(cf) * pg * op.m;

@@ -1,0 +1,169 @@
+// Complex-FP32 GEMM core on MFMA for one wave (device code, header only).
+//
+// A wave accumulates an (16 RB) x (16 CB) block of  sum_k A[i][k] B[k][j]
+// over a list of k-blocks of 16.  Inside a k-block, lane (g = lane / 16,
+// c = lane % 16) contracts k = k0 + 4g + s at MFMA step s (s = 0..3), which
+// makes every operand layout below load whole cache lines per 16 lanes, and
+// the four k of a lane contiguous where the layout allows it.
+//
+// Complex products (v_mfma_f32_16x16x4f32, FP32 accumulation) take either
+// four real products (M3 = false):  re = ar br - ai bi,  im = ar bi + ai br,
+// or three (M3 = true, the 3M form):
+//   p1 = ar br,  p2 = ai bi,  p3 = (ar + ai)(br + bi);   re = p1 - p2,  im = p3 - p1 - p2
+// The 3M form saves a quarter of the MFMA work; its imaginary part carries
+// the rounding of |a||b|-sized terms, so it serves self-correcting iterations
+// (Newton-Schulz) but not the sweeps, whose accuracy sets the refinement count.
+//
+// Operand layouts (lda / ldb in complex elements):
+//   A_COL    A[i][k] = P[k lda + i]          column-major (inverse pages, Schur blocks)
+//   A_CONJT  A[i][k] = conj(P[i lda + k])    conjugate transpose of a column-major P
+//   B_KCOL   B[k][j] = Q[j ldb + k]          column-major (k contiguous)
+//   B_ROW    B[k][j] = Q[k ldb + j]          row-major (fields: RHS fastest)
+//
+// The k loop keeps two k-blocks of operands in flight (three register sets
+// rotated by unrolling, no register copies), so a wave waits only for data
+// requested two k-blocks earlier.
+#pragma once
+#include "wust_common.h"
+
+namespace wust {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+enum { A_COL = 0, A_CONJT = 1 };
+enum { B_KCOL = 0, B_ROW = 1 };
+
+template <int RB, int CB, int ALAY, int BLAY, bool M3>
+struct CgemmWave {
+    // operands of one k-block: a[rb][s], b[cb][s]
+    struct Ops {
+        cf a[RB][4], b[CB][4];
+    };
+    v4f p1[RB][CB], p2[RB][CB], p3[RB][CB];   // p2 unused without M3
+
+    __device__ __forceinline__ void zero() {
+#pragma unroll
+        for (int x = 0; x < RB; ++x)
+#pragma unroll
+            for (int y = 0; y < CB; ++y) p1[x][y] = p2[x][y] = p3[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
+    }
+
+    // A points at A[i0][0] of the layout, B at B[0][j0].
+    __device__ __forceinline__ static void load(Ops &o, const cf *A, int64_t lda, const cf *B, int64_t ldb, int k0,
+                                                int g, int c) {
+        const int k = k0 + 4 * g;
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const int i = 16 * rb + c;
+            if (ALAY == A_COL) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) o.a[rb][s] = A[(int64_t)(k + s) * lda + i];
+            } else {
+                const float4 *p = reinterpret_cast<const float4 *>(A + (int64_t)i * lda + k);
+                const float4 u0 = p[0], u1 = p[1];
+                o.a[rb][0] = mk(u0.x, -u0.y);
+                o.a[rb][1] = mk(u0.z, -u0.w);
+                o.a[rb][2] = mk(u1.x, -u1.y);
+                o.a[rb][3] = mk(u1.z, -u1.w);
+            }
+        }
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+            const int j = 16 * cb + c;
+            if (BLAY == B_KCOL) {
+                const float4 *p = reinterpret_cast<const float4 *>(B + (int64_t)j * ldb + k);
+                const float4 u0 = p[0], u1 = p[1];
+                o.b[cb][0] = mk(u0.x, u0.y);
+                o.b[cb][1] = mk(u0.z, u0.w);
+                o.b[cb][2] = mk(u1.x, u1.y);
+                o.b[cb][3] = mk(u1.z, u1.w);
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) o.b[cb][s] = B[(int64_t)(k + s) * ldb + j];
+            }
+        }
+    }
+
+    __device__ __forceinline__ void mma(const Ops &o) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (M3) {
+                float as[RB], bs[CB];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) as[rb] = o.a[rb][s].re + o.a[rb][s].im;
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb) bs[cb] = o.b[cb][s].re + o.b[cb][s].im;
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < CB; ++cb) {
+                        p1[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[rb][s].re, o.b[cb][s].re, p1[rb][cb], 0, 0, 0);
+                        p2[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[rb][s].im, o.b[cb][s].im, p2[rb][cb], 0, 0, 0);
+                        p3[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(as[rb], bs[cb], p3[rb][cb], 0, 0, 0);
+                    }
+            } else {
+                // p1 accumulates the real part, p3 the imaginary part
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < CB; ++cb) {
+                        const cf x = o.a[rb][s], y = o.b[cb][s];
+                        p1[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.re, p1[rb][cb], 0, 0, 0);
+                        p3[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.im, p3[rb][cb], 0, 0, 0);
+                        p1[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-x.im, y.im, p1[rb][cb], 0, 0, 0);
+                        p3[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.im, y.re, p3[rb][cb], 0, 0, 0);
+                    }
+            }
+        }
+    }
+
+    // Accumulate over k-blocks kb = kb0, kb0 + step, ... < kb_end.  Loads are
+    // issued unconditionally (indices past the last k-block re-read the last
+    // one and are not used), so the wait counts before each MFMA block are
+    // static; only the MFMA blocks are guarded, by wave-uniform branches.
+    __device__ __forceinline__ void run(const cf *A, int64_t lda, const cf *B, int64_t ldb, int kb0, int kb_end,
+                                        int step, int g, int c) {
+        const int nk = (kb_end - kb0 + step - 1) / step;
+        if (nk <= 0) return;
+        const int last = kb0 + (nk - 1) * step;
+        auto k0 = [&](int t) {
+            const int kb = kb0 + t * step;
+            return 16 * (kb < last ? kb : last);
+        };
+        // sched_barrier(0) keeps the scheduler from moving loads across the
+        // MFMA phases, which would make the wait counts cover later loads.
+        Ops o0, o1, o2;
+        load(o0, A, lda, B, ldb, k0(0), g, c);
+        load(o1, A, lda, B, ldb, k0(1), g, c);
+        for (int t = 0; t < nk; t += 3) {
+            load(o2, A, lda, B, ldb, k0(t + 2), g, c);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(o0);
+            __builtin_amdgcn_sched_barrier(0);
+            load(o0, A, lda, B, ldb, k0(t + 3), g, c);
+            __builtin_amdgcn_sched_barrier(0);
+            if (t + 1 < nk) mma(o1);
+            __builtin_amdgcn_sched_barrier(0);
+            load(o1, A, lda, B, ldb, k0(t + 4), g, c);
+            __builtin_amdgcn_sched_barrier(0);
+            if (t + 2 < nk) mma(o2);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    // Complex result of block (rb, cb): lane holds rows 4g + r (r = 0..3), column c.
+    __device__ __forceinline__ void result(int rb, int cb, float (&re)[4], float (&im)[4]) const {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (M3) {
+                re[r] = p1[rb][cb][r] - p2[rb][cb][r];
+                im[r] = p3[rb][cb][r] - p1[rb][cb][r] - p2[rb][cb][r];
+            } else {
+                re[r] = p1[rb][cb][r];
+                im[r] = p3[rb][cb][r];
+            }
+        }
+    }
+};
+
+}  // namespace wust

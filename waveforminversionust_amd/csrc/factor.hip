@@ -27,6 +27,7 @@
 #include <cstring>
 #include <utility>
 
+#include "cgemm.h"
 #include "engine.h"
 #include "profile.h"
 
@@ -41,7 +42,7 @@ struct SchurTerm {
     const cf *Pn, *Pc, *Ps;      // left tridiagonal factor, row coefficients
     const cf *Qn, *Qc, *Qs;      // right tridiagonal factor, row coefficients
 };
-constexpr int NS_KMAX = 8;  // Newton-Schulz updates per block, upper limit
+constexpr int NS_KMAX = 8;  // Newton-Schulz rounds per block, upper limit
 
 struct SchurTask {
     const cf *Dn, *Dc, *Ds;
@@ -106,7 +107,6 @@ struct GjParams {
     float *pivot_ratio;   // per task: running min |pivot| / max |entry below it| in the diagonal block
 };
 
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 struct GjShared {
     cf Pd[NB][NB + 1];     // P, row-major [r][c]
@@ -397,32 +397,37 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(GjParams p) {
 // ---------------------------------------------------------------------------
 // Newton-Schulz block inverse.  Consecutive Schur blocks of one chain differ
 // little away from the PML (T_j X_{j-1} is close to I), so X_j is reached from
-// X_0 = X_{j-1} by
-//   E_k = I - T X_k,   X_{k+1} = X_k + X_k E_k      (E_{k+1} = E_k^2)
-// on MFMA instead of a serial pivot chain.  ns_gemm_kernel computes one such
-// product for up to two chains:  out = C0 + sgn * A B  (C0 = I when null), all
-// npad x npad column-major, optionally accumulating ||out||_F^2.  The
-// iteration count is decided on the device: an update is the last one when
-// ||E_k||_F^2 <= thr2 (the residual after it is bounded by ||E_k||_F^2) or when
-// the host-side iteration budget is spent; launches after the last update of
-// a chain return at once.
+// X_0 = X_{j-1} on MFMA instead of by a serial pivot chain.  Round r of a
+// block, with iterate X_r and residual E_r = I - T X_r:
+//   ||E_r||_F <= q            last update, quadratic:  X_{r+1} = X_r (I + E_r)
+//   q < ||E_r||_F <= c        last update, cubic:      X_{r+1} = X_r (I + E_r + E_r^2)
+//   otherwise                 X_{r+1} = X_r (I + E_r), then E_{r+1} = I - T X_{r+1}
+// (I - T X_{r+1} = E_r^2, resp. E_r^3, so the residual after the last update
+// is bounded by ||E_r||_F^2, resp. ||E_r||_F^3; q and c put that bound at the
+// FP32 floor of these blocks).  The last round of the host's budget is final
+// whatever the norm.  Each round is three launches -- residual, cubic
+// operand, update -- and each launch decides on the device, from the norms
+// of earlier launches, whether it has work; launches after a block's last
+// update return at once.  The residual sets the accuracy floor and uses four
+// real products per complex product; the products with E_r (small) use 3M.
 // ---------------------------------------------------------------------------
+enum NsKind { NS_RESIDUAL = 0, NS_CUBIC = 1, NS_UPDATE = 2 };
+
 struct NsTask {
-    const cf *A, *B;
-    const cf *C0;             // null: identity
-    cf *out_next;             // output when further updates follow
-    cf *out_final;            // output of the last update (null: out_next)
-    float *norm_out;          // non-null: accumulates ||out||_F^2
-    const float *done_norm;   // non-null: the chain is finished when *done_norm <= thr2
-    const float *cur_norm;    // non-null: this update is the last when *cur_norm <= thr2
-    int force_final;          // this update is the last regardless
-    float sgn;
+    const cf *T;      // Schur block
+    const cf *X;      // iterate X_r
+    cf *E;            // residual E_r
+    cf *M;            // cubic operand E_r + E_r^2
+    cf *page;         // X_{r+1} when this round is the last
+    cf *xnext;        // X_{r+1} otherwise
+    float *norms;     // ||E_r||_F^2 per round (cleared by the Schur launch)
 };
 struct NsParams {
     NsTask task[2];
     int ntasks, npad;
-    float thr2;        // ||E_k||_F^2 below which one more update reaches the FP32 floor
-    float accept;      // largest acceptable bound on ||I - T X||_F after the last update
+    int kind, r, R;   // launch kind, round, round budget
+    float q2, c2;     // q^2, c^2 (squared Frobenius norms)
+    float accept;     // largest acceptable residual bound after the last update
     uint32_t *status;
     float *resid_max;  // running max of that bound over the factor (float bits, atomicMax)
 };
@@ -443,12 +448,11 @@ __device__ __forceinline__ void ns_tile(int r, int T, int &ti, int &tj) {
     tj = c0[q] + r % cols[q];
 }
 
-// One 32x32 output tile per workgroup; the four waves split the reduction
-// (k-blocks of 16 dealt round-robin) and each computes the whole tile with
-// 2x2 MFMA 16x16x4 blocks.  Inside a k-block, lane (g = lane / 16, c = lane % 16)
-// contracts k = k0 + 4g + s at MFMA step s, so its B operand for the four
-// steps is four consecutive complex values of one column (two 16-byte
-// loads) and the four lanes of a column read one 128-byte line.
+// One 32x32 output tile of  out = C0 + sgn A B  per workgroup (C0 = I when
+// null); the four waves split the reduction (k-blocks of 16 dealt
+// round-robin) and each accumulates the whole tile as 2x2 blocks of 16x16
+// (CgemmWave); the partial tiles are summed through LDS.
+template <bool M3>
 __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
     const int T = p.npad / NB, per_task = T * T, G = p.ntasks * per_task;
     const int per_xcd = (G + 7) / 8;
@@ -456,9 +460,26 @@ __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
     if (L >= G) return;
     const int task = L / per_task;
     const NsTask &tk = p.task[task];
-    if (tk.done_norm && *tk.done_norm <= p.thr2) return;
-    const bool last = tk.force_final || (tk.cur_norm && *tk.cur_norm <= p.thr2);
-    cf *out = (last && tk.out_final) ? tk.out_final : tk.out_next;
+    // a block whose previous round was its last is finished
+    if (p.r > 0 && tk.norms[p.r - 1] <= p.c2) return;
+    const cf *A, *B, *C0;
+    float sgn;
+    cf *out;
+    float *norm_out = nullptr;
+    bool last = false, cubic = false;
+    if (p.kind == NS_RESIDUAL) {
+        A = tk.T; B = tk.X; C0 = nullptr; sgn = -1.f; out = tk.E; norm_out = tk.norms + p.r;
+    } else {
+        const float n = tk.norms[p.r];
+        last = (n <= p.c2) || (p.r == p.R - 1);
+        cubic = last && (n > p.q2);
+        if (p.kind == NS_CUBIC) {
+            if (!cubic) return;
+            A = tk.E; B = tk.E; C0 = tk.E; sgn = 1.f; out = tk.M;
+        } else {
+            A = tk.X; B = cubic ? tk.M : tk.E; C0 = tk.X; sgn = 1.f; out = last ? tk.page : tk.xnext;
+        }
+    }
     int ti, tj;
     ns_tile(L % per_task, T, ti, tj);
 
@@ -466,65 +487,15 @@ __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
     __shared__ float wsum[4];
     const int ld = p.npad, i0 = ti * NB, j0 = tj * NB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-    const cf *A = tk.A, *B = tk.B;
-
-    v4f dr[2][2], di[2][2];
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) dr[x][y] = di[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
-
-    const int KB = ld / 16;
-    cf a[2][4], bm[2][4], an[2][4], bn[2][4];
-    auto load = [&](int kb, cf (&av)[2][4], cf (&bv)[2][4]) {
-        const int k0 = kb * 16 + 4 * g;
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-            for (int s = 0; s < 4; ++s) av[rb][s] = A[(int64_t)(k0 + s) * ld + i0 + 16 * rb + c];
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
-            const float4 *src = reinterpret_cast<const float4 *>(B + (int64_t)(j0 + 16 * cb + c) * ld + k0);
-            const float4 u0 = src[0], u1 = src[1];
-            bv[cb][0] = mk(u0.x, u0.y);
-            bv[cb][1] = mk(u0.z, u0.w);
-            bv[cb][2] = mk(u1.x, u1.y);
-            bv[cb][3] = mk(u1.z, u1.w);
-        }
-    };
-    int kb = wave;
-    if (kb < KB) load(kb, a, bm);
-    while (kb < KB) {
-        const int kn = kb + 4;
-        if (kn < KB) load(kn, an, bn);
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                for (int cb = 0; cb < 2; ++cb) {
-                    const cf x = a[rb][s], y = bm[cb][s];
-                    dr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.re, dr[rb][cb], 0, 0, 0);
-                    dr[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(-x.im, y.im, dr[rb][cb], 0, 0, 0);
-                    di[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.im, di[rb][cb], 0, 0, 0);
-                    di[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(x.im, y.re, di[rb][cb], 0, 0, 0);
-                }
-        if (kn < KB) {
-#pragma unroll
-            for (int x = 0; x < 2; ++x)
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    a[x][s] = an[x][s];
-                    bm[x][s] = bn[x][s];
-                }
-        }
-        kb = kn;
-    }
+    CgemmWave<2, 2, A_COL, B_KCOL, M3> acc;
+    acc.zero();
+    acc.run(A + i0, ld, B + (int64_t)j0 * ld, ld, wave, ld / 16, 4, g, c);
 
     // reduce the four k-partials: wave w finishes block (w / 2, w % 2)
 #pragma unroll
     for (int blk = 0; blk < 4; ++blk) {
-        const v4f &r = dr[blk >> 1][blk & 1], &im = di[blk >> 1][blk & 1];
+        float r[4], im[4];
+        acc.result(blk >> 1, blk & 1, r, im);
         red[wave][blk][0][lane] = make_float4(r[0], r[1], r[2], r[3]);
         red[wave][blk][1][lane] = make_float4(im[0], im[1], im[2], im[3]);
     }
@@ -541,8 +512,8 @@ __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
     const int row = i0 + 16 * rb + 4 * g, col = j0 + 16 * cb + c;
     const int64_t off = (int64_t)col * ld + row;
     cf v[4];
-    if (tk.C0) {
-        const float4 *src = reinterpret_cast<const float4 *>(tk.C0 + off);
+    if (C0) {
+        const float4 *src = reinterpret_cast<const float4 *>(C0 + off);
         const float4 u0 = src[0], u1 = src[1];
         v[0] = mk(u0.x, u0.y);
         v[1] = mk(u0.z, u0.w);
@@ -555,21 +526,23 @@ __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
     float n2 = 0.f;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        v[r] = mk(v[r].re + tk.sgn * sr[r], v[r].im + tk.sgn * si[r]);
+        v[r] = mk(v[r].re + sgn * sr[r], v[r].im + sgn * si[r]);
         n2 += abs2(v[r]);
     }
     float4 *dst = reinterpret_cast<float4 *>(out + off);
     dst[0] = make_float4(v[0].re, v[0].im, v[1].re, v[1].im);
     dst[1] = make_float4(v[2].re, v[2].im, v[3].re, v[3].im);
 
-    if (tk.norm_out) {
+    if (norm_out) {
         for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o, 64);
         if (lane == 0) wsum[wave] = n2;
         lds_barrier();
-        if (threadIdx.x == 0) atomicAdd(tk.norm_out, wsum[0] + wsum[1] + wsum[2] + wsum[3]);
+        if (threadIdx.x == 0) atomicAdd(norm_out, wsum[0] + wsum[1] + wsum[2] + wsum[3]);
     }
-    if (last && tk.cur_norm && L % per_task == 0 && threadIdx.x == 0) {
-        const float bound = *tk.cur_norm;   // ||I - T X_{k+1}||_F <= ||E_k||_F^2
+    if (p.kind == NS_UPDATE && last && L % per_task == 0 && threadIdx.x == 0) {
+        // residual bound after this update: ||E_r||^2 (quadratic) or ||E_r||^3 (cubic)
+        const float n = tk.norms[p.r];
+        const float bound = cubic ? n * sqrtf(n) : n;
         if (!(bound <= p.accept)) atomicOr(p.status, (uint32_t)ST_INVERSE_UNCONVERGED);
         if (bound >= 0.f) atomicMax(reinterpret_cast<int *>(p.resid_max), __float_as_int(bound));
     }
@@ -613,11 +586,12 @@ void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s) {
 
 namespace {
 
-// Newton-Schulz controls.  An update is the last one once ||E_k||_F <= 1e-3
-// (the residual after it is <= 1e-6, the FP32 floor of these blocks); a
-// block whose bound after its last update exceeds 1e-4 sends the whole
-// factor back to Gauss-Jordan.
-constexpr float NS_THR2 = 1e-6f;
+// Newton-Schulz controls: the residual after a block's last update is
+// brought to 2e-6 (the FP32 floor of these blocks): quadratic when
+// ||E||_F^2 <= 2e-6, cubic when ||E||_F^3 <= 2e-6.  A block whose bound after
+// its last update exceeds 1e-4 sends the whole factor back to Gauss-Jordan.
+constexpr float NS_Q2 = 2e-6f;
+constexpr float NS_C2 = 1.5874e-4f;   // (2e-6)^(2/3)
 constexpr float NS_ACCEPT = 1e-4f;
 
 struct FactorRun {
@@ -628,10 +602,10 @@ struct FactorRun {
     int ns_blocks = 0;
     int update_cap = NS_KMAX;
 
-    // Newton-Schulz update budget of chain step `st`: the PML columns at
-    // the start of each chain change fastest.
+    // Newton-Schulz round budget of chain step `st`: the PML columns at the
+    // start of each chain change fastest.
     int ns_budget(int st) const {
-        const int b = st < pml_cells + 8 ? NS_KMAX : 4;
+        const int b = st < pml_cells + 8 ? NS_KMAX : 2;
         return b < update_cap ? b : update_cap;
     }
 
@@ -691,47 +665,50 @@ struct FactorRun {
             }
         }
         if (!ns.empty()) {
-            const int K = ns_budget(st);
+            const int R = ns_budget(st);
             const int T = ld / NB;
             const int G = (int)ns.size() * T * T;
             const int grid_ns = 8 * ((G + 7) / 8);
-            for (int k = 0; k < K; ++k) {
-                NsParams pe{}, px{};
-                pe.ntasks = px.ntasks = (int)ns.size();
-                pe.npad = px.npad = ld;
-                pe.thr2 = px.thr2 = NS_THR2;
-                pe.accept = px.accept = NS_ACCEPT;
-                pe.status = px.status = op.status;
-                pe.resid_max = px.resid_max = reinterpret_cast<float *>(op.status + 4);
+            NsParams np{};
+            np.ntasks = (int)ns.size();
+            np.npad = ld;
+            np.R = R;
+            np.q2 = NS_Q2;
+            np.c2 = NS_C2;
+            np.accept = NS_ACCEPT;
+            np.status = op.status;
+            np.resid_max = reinterpret_cast<float *>(op.status + 4);
+            auto set_round = [&](int r) {
+                np.r = r;
                 for (int t = 0; t < (int)ns.size(); ++t) {
                     const int c = ns[t];
                     const int j = steps[c].block;
-                    const cf *T0 = op.work + (int64_t)c * 2 * pg;     // Schur block
-                    cf *E = op.work + (int64_t)c * 2 * pg + pg;
-                    cf *xb = op.nswork + (int64_t)c * 2 * pg;
-                    const cf *Xk = (k == 0) ? op.page(steps[c].kind == 0 ? j - 1 : j + 1) : xb + ((k - 1) % 2) * pg;
-                    float *nrm = op.ns_norm + c * NS_KMAX;
-                    NsTask &te = pe.task[t];
-                    te.A = T0;
-                    te.B = Xk;
-                    te.C0 = nullptr;
-                    te.sgn = -1.f;
-                    te.out_next = E;
-                    te.norm_out = nrm + k;
-                    te.done_norm = (k > 0) ? nrm + k - 1 : nullptr;
-                    NsTask &tx = px.task[t];
-                    tx.A = Xk;
-                    tx.B = E;
-                    tx.C0 = Xk;
-                    tx.sgn = 1.f;
-                    tx.out_next = xb + (k % 2) * pg;
-                    tx.out_final = op.page(j);
-                    tx.done_norm = (k > 0) ? nrm + k - 1 : nullptr;
-                    tx.cur_norm = nrm + k;
-                    tx.force_final = (k == K - 1);
+                    NsTask &tk = np.task[t];
+                    cf *w = op.work + (int64_t)c * 2 * pg;       // [T | E]
+                    cf *xb = op.nswork + (int64_t)c * 3 * pg;    // [X_a | X_b | M]
+                    tk.T = w;
+                    tk.E = w + pg;
+                    tk.M = xb + 2 * pg;
+                    tk.X = (r == 0) ? op.page(steps[c].kind == 0 ? j - 1 : j + 1) : xb + ((r - 1) % 2) * pg;
+                    tk.xnext = xb + (r % 2) * pg;
+                    tk.page = op.page(j);
+                    tk.norms = op.ns_norm + c * NS_KMAX;
                 }
-                launch_k(PC_NS_GEMM, ns_gemm_kernel, dim3(grid_ns), dim3(256), 0, s, pe);
-                launch_k(PC_NS_GEMM, ns_gemm_kernel, dim3(grid_ns), dim3(256), 0, s, px);
+            };
+            set_round(0);
+            np.kind = NS_RESIDUAL;
+            launch_k(PC_NS_GEMM, ns_gemm_kernel<false>, dim3(grid_ns), dim3(256), 0, s, np);
+            for (int r = 0; r < R; ++r) {
+                set_round(r);
+                np.kind = NS_CUBIC;
+                launch_k(PC_NS_GEMM, ns_gemm_kernel<true>, dim3(grid_ns), dim3(256), 0, s, np);
+                np.kind = NS_UPDATE;
+                launch_k(PC_NS_GEMM, ns_gemm_kernel<true>, dim3(grid_ns), dim3(256), 0, s, np);
+                if (r + 1 < R) {
+                    set_round(r + 1);
+                    np.kind = NS_RESIDUAL;
+                    launch_k(PC_NS_GEMM, ns_gemm_kernel<false>, dim3(grid_ns), dim3(256), 0, s, np);
+                }
             }
             ns_blocks += (int)ns.size();
         }
@@ -759,8 +736,9 @@ void factor_engine(Operator &op, hipStream_t s) {
     WUST_HIP_CHECK(hipMemcpyAsync(op.status, init, sizeof(init), hipMemcpyHostToDevice, s));
     const int pml_cells = (int)std::ceil(op.ap.L / op.ap.dx);
     FactorRun run{op, s, op.factor_method == FACTOR_AUTO, pml_cells};
-    // WUST_NS_MAX_UPDATES caps the Newton-Schulz updates per block (tuning;
-    // a cap too small for the PML blocks exercises the Gauss-Jordan fallback)
+    // WUST_NS_MAX_UPDATES caps the Newton-Schulz rounds (one update each) per
+    // block (tuning; a cap too small for the PML blocks exercises the
+    // Gauss-Jordan fallback)
     if (const char *cap = std::getenv("WUST_NS_MAX_UPDATES")) {
         const int c = std::atoi(cap);
         if (c >= 1 && c <= NS_KMAX) run.update_cap = c;

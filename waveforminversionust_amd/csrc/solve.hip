@@ -15,14 +15,13 @@
 //            (H^H has U_{j-1}^H below and L_{j+1}^H above the diagonal).
 // Layout of every field block: [npad rows][Rpad RHS], RHS fastest, so the
 // coupling loads and the output stores are coalesced across RHS.
+#include "cgemm.h"
 #include "engine.h"
 #include "profile.h"
 
 namespace wust {
 
 namespace {
-
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 struct SweepTerm {
     const cf *cn, *cc, *cs;   // tridiagonal row coefficients (block of the coupling)
@@ -31,7 +30,7 @@ struct SweepTerm {
 };
 struct SweepTask {
     const cf *X;              // inverse page
-    int xconj;                // use X^H
+    int xconj;                // use X^H (selects the kernel instantiation)
     const cf *src;            // source block or null
     SweepTerm t[2];
     int nterms;
@@ -78,99 +77,69 @@ __global__ void __launch_bounds__(256) sweep_operand_kernel(SweepParams p) {
     tk.B[rem] = operand_b(tk, k, r, p.npad, p.Rpad);
 }
 
-// GEMM pass: out (+)= op(X) B on a 16 x 16 tile (16 block rows x 16 RHS).
-// The 4 waves of a WG split the K range into quarters and stream their MFMA
-// operands straight from global memory into registers in the MFMA lane layout
-// (A: lane (li, kq) = X[row li][k kq], B: lane = B[k kq][col li]) -- no operand
-// is shared between waves, so no LDS is needed in the main loop.  Operands are
-// prefetched in two register batches of DB sub-steps (4 k each).  The four
-// partial tiles are summed through LDS at the end.  16 x 16 tiles give
-// 2 * 36 * 16 = 1152 WGs at the 560^2 x 256-RHS size (4.5 per CU), which
-// balances the 256 CUs far better than 32 x 32 tiles (288 WGs).
-constexpr int DB = 6;   // sub-steps per prefetch batch (24 k)
-
-// 5 waves per SIMD keep all 1152 WGs of a 560^2 x 256-RHS step resident at once.
-__global__ void __launch_bounds__(256, 5) sweep_gemm_kernel(SweepParams p) {
+// GEMM pass: out (+)= op(X) B, op(X) = X or X^H, on tiles of SM x 16 (SM = 16 RB
+// block rows x 16 RHS).  The 4 waves of a WG deal the k-blocks of 16
+// round-robin and each accumulates the whole tile (CgemmWave: four real
+// products per complex product, two k-blocks of operands in flight, straight
+// from global memory in the MFMA lane layout); the partial tiles are summed
+// through LDS.  Tiles
+// are dealt to XCDs in contiguous runs (blockIdx % 8 is the XCD), so one
+// XCD's L2 serves a band of rows of X and a band of RHS columns.
+template <int RB, int ALAY>
+__global__ void __launch_bounds__(256) sweep_gemm_kernel(SweepParams p) {
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
     const int L = xcd * p.per_xcd + local;
     if (L >= p.units) return;
     const int npad = p.npad, Rpad = p.Rpad;
-    const int Mb = npad / 16, Nb = Rpad / 16;
-    const int nb = L % Nb;
-    const int mb = (L / Nb) % Mb;
+    const int Mb = npad / (16 * RB), Nb = Rpad / 16;
+    const int mb = L % Mb;
+    const int nb = (L / Mb) % Nb;
     const int task = L / (Nb * Mb);
     const SweepTask &tk = p.task[task];
-    const int i0 = mb * 16, j0 = nb * 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, kq = lane >> 4;
-    const int kquarter = npad / 4;           // multiple of 8 (npad is a multiple of 32)
-    const int nsub = kquarter / 4;           // sub-steps of 4 k in this wave's quarter
-    const int kbeg = wave * kquarter;
+    const int i0 = mb * 16 * RB, j0 = nb * 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
 
-    // lane-resolved operand pointers; sub-step s reads element offset s*4*stride
-    const cf *ap;
-    int64_t astep;
-    if (!tk.xconj) {
-        ap = tk.X + (int64_t)(i0 + li) + (int64_t)npad * (kbeg + kq);
-        astep = 4 * (int64_t)npad;
-    } else {
-        ap = tk.X + (int64_t)(kbeg + kq) + (int64_t)npad * (i0 + li);
-        astep = 4;
-    }
-    const cf *bp = tk.B + (int64_t)(kbeg + kq) * Rpad + j0 + li;
-    const int64_t bstep = 4 * (int64_t)Rpad;
-    const float asgn = tk.xconj ? -1.f : 1.f;   // conj(X) for the adjoint
+    CgemmWave<RB, 1, ALAY, B_ROW, false> acc;
+    acc.zero();
+    const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
+    acc.run(A, npad, tk.B + j0, Rpad, wave, npad / 16, 4, g, c);
 
-    v4f ar = {0.f, 0.f, 0.f, 0.f}, ai = ar;
-    cf a0[DB], b0[DB], a1[DB], b1[DB];
-    auto load = [&](int s0, cf (&ra)[DB], cf (&rb)[DB]) {
-#pragma unroll
-        for (int u = 0; u < DB; ++u) {
-            const int sidx = s0 + u;
-            if (sidx < nsub) {
-                ra[u] = ap[sidx * astep];
-                rb[u] = bp[sidx * bstep];
-            } else {
-                ra[u] = mk(0.f, 0.f);
-                rb[u] = mk(0.f, 0.f);
-            }
-        }
-    };
-    auto mma = [&](const cf (&ra)[DB], const cf (&rb)[DB]) {
-#pragma unroll
-        for (int u = 0; u < DB; ++u) {
-            const float are = ra[u].re, aim = asgn * ra[u].im;
-            ar = __builtin_amdgcn_mfma_f32_16x16x4f32(are, rb[u].re, ar, 0, 0, 0);
-            ai = __builtin_amdgcn_mfma_f32_16x16x4f32(are, rb[u].im, ai, 0, 0, 0);
-            ar = __builtin_amdgcn_mfma_f32_16x16x4f32(-aim, rb[u].im, ar, 0, 0, 0);
-            ai = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, rb[u].re, ai, 0, 0, 0);
-        }
-    };
-    load(0, a0, b0);
-    for (int s0 = 0; s0 < nsub; s0 += 2 * DB) {
-        load(s0 + DB, a1, b1);
-        mma(a0, b0);
-        if (s0 + DB >= nsub) break;
-        load(s0 + 2 * DB, a0, b0);
-        mma(a1, b1);
-    }
-    __shared__ cf red[3][16][17];
+    // waves 1..3 park their partial tiles; wave 0 sums and stores
+    __shared__ float4 red[3][RB][2][64];
     if (wave > 0) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) red[wave - 1][4 * kq + r][li] = mk(ar[r], ai[r]);
+        for (int rb = 0; rb < RB; ++rb) {
+            float re[4], im[4];
+            acc.result(rb, 0, re, im);
+            red[wave - 1][rb][0][lane] = make_float4(re[0], re[1], re[2], re[3]);
+            red[wave - 1][rb][1][lane] = make_float4(im[0], im[1], im[2], im[3]);
+        }
     }
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 4 * kq + r;
-            cf sacc = mk(ar[r], ai[r]) + red[0][row][li] + red[1][row][li] + red[2][row][li];
-            cf *o = tk.out + (int64_t)(i0 + row) * Rpad + j0 + li;
-            if (tk.accumulate) sacc = sacc + *o;
-            *o = sacc;
+        for (int rb = 0; rb < RB; ++rb) {
+            float re[4], im[4];
+            acc.result(rb, 0, re, im);
+#pragma unroll
+            for (int w = 0; w < 3; ++w) {
+                const float4 u = red[w][rb][0][lane], v = red[w][rb][1][lane];
+                re[0] += u.x; re[1] += u.y; re[2] += u.z; re[3] += u.w;
+                im[0] += v.x; im[1] += v.y; im[2] += v.z; im[3] += v.w;
+            }
+            // lane holds rows i0 + 16 rb + 4 g + r of RHS column j0 + c
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                cf *o = tk.out + (int64_t)(i0 + 16 * rb + 4 * g + r) * Rpad + j0 + c;
+                cf v = mk(re[r], im[r]);
+                if (tk.accumulate) v = v + *o;
+                *o = v;
+            }
         }
     }
 }
+
+constexpr int SWEEP_RB = 2;   // 32-row tiles: 2 * 18 * 16 = 576 WGs at 560^2 x 256 RHS
 
 }  // namespace
 
@@ -199,9 +168,12 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *scratch, int R
         sp.Rpad = Rpad;
         const int64_t nops = (int64_t)sp.ntasks * blk;
         launch_k(PC_SWEEP_OPERAND, sweep_operand_kernel, dim3((unsigned)((nops + 255) / 256)), dim3(256), 0, s, sp);
-        sp.units = sp.ntasks * (npad / 16) * (Rpad / 16);
+        sp.units = sp.ntasks * (npad / (16 * SWEEP_RB)) * (Rpad / 16);
         sp.per_xcd = (sp.units + 7) / 8;
-        launch_k(PC_SWEEP_GEMM, sweep_gemm_kernel, dim3(8 * sp.per_xcd), dim3(256), 0, s, sp);
+        if (adjoint)
+            launch_k(PC_SWEEP_GEMM, sweep_gemm_kernel<SWEEP_RB, A_CONJT>, dim3(8 * sp.per_xcd), dim3(256), 0, s, sp);
+        else
+            launch_k(PC_SWEEP_GEMM, sweep_gemm_kernel<SWEEP_RB, A_COL>, dim3(8 * sp.per_xcd), dim3(256), 0, s, sp);
     };
     auto base = [&](int j, bool with_src, bool acc) {
         SweepTask tk{};
