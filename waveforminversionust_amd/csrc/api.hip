@@ -212,8 +212,11 @@ struct wust_helmholtz {
         wust_solve_stats st{};
         const int64_t cnt = field_elems(Rpad);
         double *nb2 = norms, *nr2 = norms + Rpad;
+        // one pass: U_new = U_old + D, R = b - H U_new, per-RHS |R|^2 (and |b|^2
+        // on the first pass); returns the max over RHS of |R| / |b|
+        bool first_pass = true;
         auto residual_pass = [&](const cd *u_old, const cf *d, cd *u_new, cf *res32) {
-            WUST_HIP_CHECK(hipMemsetAsync(nr2, 0, sizeof(double) * Rpad, s));
+            WUST_HIP_CHECK(hipMemsetAsync(first_pass ? nb2 : nr2, 0, sizeof(double) * Rpad * (first_pass ? 2 : 1), s));
             ResidualArgs a{};
             a.w64 = op.w64;
             a.u_old = u_old;
@@ -222,12 +225,14 @@ struct wust_helmholtz {
             a.u_new = u_new;
             a.res32 = res32;
             a.norm2 = nr2;
+            a.bnorm2 = first_pass ? nb2 : nullptr;
             a.n = op.n;
             a.m = op.m;
             a.npad = op.npad;
             a.Rpad = Rpad;
             a.adjoint = adjoint ? 1 : 0;
             launch_residual(a, s);
+            first_pass = false;
             std::vector<double> h(2 * (size_t)Rpad);
             WUST_HIP_CHECK(hipMemcpyAsync(h.data(), norms, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
             WUST_HIP_CHECK(hipStreamSynchronize(s));
@@ -237,19 +242,6 @@ struct wust_helmholtz {
                 else if (h[Rpad + r] > 0) worst = INFINITY;
             return worst;
         };
-        // right-hand-side norms
-        {
-            WUST_HIP_CHECK(hipMemsetAsync(nb2, 0, sizeof(double) * Rpad, s));
-            ResidualArgs a{};
-            a.w64 = op.w64;
-            a.b64 = b;
-            a.norm2 = nb2;
-            a.n = op.n;
-            a.m = op.m;
-            a.npad = op.npad;
-            a.Rpad = Rpad;
-            launch_residual(a, s);
-        }
         if (op.mode == WUST_MODE_PLAIN) {
             solve64_to_plain32(b, p_src, op.m, op.npad, R, Rpad, s);
             solve_plain(op, plain, p_src, p_u, p_tmp, R, adjoint, s);
@@ -266,8 +258,10 @@ struct wust_helmholtz {
             if (want_residual) st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr);
             return st;
         }
-        // refinement: U_new = U_old + D; R = b - H U_new; D = solve(R)
-        cd *bufs[2] = {out, t64 == out ? u64 : t64};
+        // refinement: U_new = U_old + D; R = b - H U_new; D = solve(R).  The
+        // iterates alternate between two buffers, ordered so that the usual
+        // single correction ends in `out` without a copy.
+        cd *bufs[2] = {t64 == out ? u64 : t64, out};
         const cd *u_old = nullptr;
         int cur = 0;
         int it = 0;

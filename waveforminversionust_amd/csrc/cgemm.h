@@ -20,9 +20,9 @@
 //   B_KCOL   B[k][j] = Q[j ldb + k]          column-major (k contiguous)
 //   B_ROW    B[k][j] = Q[k ldb + j]          row-major (fields: RHS fastest)
 //
-// The k loop keeps two k-blocks of operands in flight (three register sets
-// rotated by unrolling, no register copies), so a wave waits only for data
-// requested two k-blocks earlier.
+// The k loop keeps DEPTH k-blocks of operands in flight (DEPTH + 1 register
+// sets rotated by unrolling, no register copies), so a wave waits only for
+// data requested DEPTH k-blocks earlier.
 #pragma once
 #include "wust_common.h"
 
@@ -33,7 +33,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 enum { A_COL = 0, A_CONJT = 1 };
 enum { B_KCOL = 0, B_ROW = 1 };
 
-template <int RB, int CB, int ALAY, int BLAY, bool M3>
+template <int RB, int CB, int ALAY, int BLAY, bool M3, int DEPTH = 2>
 struct CgemmWave {
     // operands of one k-block: a[rb][s], b[cb][s]
     struct Ops {
@@ -117,10 +117,14 @@ struct CgemmWave {
         }
     }
 
-    // Accumulate over k-blocks kb = kb0, kb0 + step, ... < kb_end.  Loads are
-    // issued unconditionally (indices past the last k-block re-read the last
-    // one and are not used), so the wait counts before each MFMA block are
-    // static; only the MFMA blocks are guarded, by wave-uniform branches.
+    // Accumulate over k-blocks kb = kb0, kb0 + step, ... < kb_end, keeping
+    // DEPTH k-blocks of operands in flight (DEPTH + 1 register sets rotated by
+    // unrolling).  Loads are issued unconditionally (indices past the last
+    // k-block re-read the last one and are not used), so the wait counts
+    // before each MFMA block are static; only the MFMA blocks are guarded, by
+    // wave-uniform branches.  sched_barrier(0) keeps the scheduler from moving
+    // loads across the MFMA phases, which would make the wait counts cover
+    // later loads.
     __device__ __forceinline__ void run(const cf *A, int64_t lda, const cf *B, int64_t ldb, int kb0, int kb_end,
                                         int step, int g, int c) {
         const int nk = (kb_end - kb0 + step - 1) / step;
@@ -130,24 +134,17 @@ struct CgemmWave {
             const int kb = kb0 + t * step;
             return 16 * (kb < last ? kb : last);
         };
-        // sched_barrier(0) keeps the scheduler from moving loads across the
-        // MFMA phases, which would make the wait counts cover later loads.
-        Ops o0, o1, o2;
-        load(o0, A, lda, B, ldb, k0(0), g, c);
-        load(o1, A, lda, B, ldb, k0(1), g, c);
-        for (int t = 0; t < nk; t += 3) {
-            load(o2, A, lda, B, ldb, k0(t + 2), g, c);
-            __builtin_amdgcn_sched_barrier(0);
-            mma(o0);
-            __builtin_amdgcn_sched_barrier(0);
-            load(o0, A, lda, B, ldb, k0(t + 3), g, c);
-            __builtin_amdgcn_sched_barrier(0);
-            if (t + 1 < nk) mma(o1);
-            __builtin_amdgcn_sched_barrier(0);
-            load(o1, A, lda, B, ldb, k0(t + 4), g, c);
-            __builtin_amdgcn_sched_barrier(0);
-            if (t + 2 < nk) mma(o2);
-            __builtin_amdgcn_sched_barrier(0);
+        Ops o[DEPTH + 1];
+#pragma unroll
+        for (int u = 0; u < DEPTH; ++u) load(o[u], A, lda, B, ldb, k0(u), g, c);
+        for (int t = 0; t < nk; t += DEPTH + 1) {
+#pragma unroll
+            for (int u = 0; u <= DEPTH; ++u) {
+                load(o[(u + DEPTH) % (DEPTH + 1)], A, lda, B, ldb, k0(t + u + DEPTH), g, c);
+                __builtin_amdgcn_sched_barrier(0);
+                if (t + u < nk) mma(o[u]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
 

@@ -15,6 +15,7 @@ struct ResidualArgs {
     cf *res32;            // B - H U_new as c64 (may be null)
     cd *res64;            // same in c128 (may be null)
     double *norm2;        // per-RHS sum |res|^2 accumulated with atomics (may be null)
+    double *bnorm2;       // per-RHS sum |b|^2 accumulated with atomics (may be null)
     int n, m, npad, Rpad, adjoint;
 };
 

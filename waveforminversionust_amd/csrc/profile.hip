@@ -1,6 +1,11 @@
 // Per-kernel-class HIP-event timing: a pool of event pairs recorded on the
 // launching stream around each bracketed launch, resolved and accumulated on
 // flush.  Process-wide, for single-threaded callers (the bench and tests).
+// Event creation costs host time comparable to a short kernel, so the pool
+// is filled when profiling is enabled and events of launches that have
+// completed are recycled before new ones are created: the launch loop then
+// runs ahead of the device as it does unprofiled.
+#include <deque>
 #include <mutex>
 #include <vector>
 
@@ -16,20 +21,45 @@ struct Pending {
 struct Profiler {
     std::mutex mu;
     uint32_t mask = 0;
-    std::vector<Pending> pending;
+    std::deque<Pending> pending;
     std::vector<hipEvent_t> pool;
     std::vector<hipEvent_t> open;   // begin events waiting for their end, per class
     double total_ms[PC_COUNT] = {};
     int64_t count[PC_COUNT] = {};
     Profiler() : open(PC_COUNT, nullptr) {}
-    hipEvent_t get() {
-        if (!pool.empty()) {
-            hipEvent_t e = pool.back();
-            pool.pop_back();
-            return e;
+    // resolve one pending pair (blocking when `wait`); false if it has not completed
+    bool resolve_front(bool wait) {
+        Pending &q = pending.front();
+        if (wait) {
+            WUST_HIP_CHECK(hipEventSynchronize(q.b));
+        } else if (hipEventQuery(q.b) != hipSuccess) {
+            return false;
         }
-        hipEvent_t e;
-        WUST_HIP_CHECK(hipEventCreate(&e));
+        float ms = 0;
+        WUST_HIP_CHECK(hipEventElapsedTime(&ms, q.a, q.b));
+        total_ms[q.cls] += ms;
+        count[q.cls] += 1;
+        pool.push_back(q.a);
+        pool.push_back(q.b);
+        pending.pop_front();
+        return true;
+    }
+    void grow(int n) {
+        for (int i = 0; i < n; ++i) {
+            hipEvent_t e;
+            WUST_HIP_CHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+    }
+    hipEvent_t get() {
+        if (pool.empty()) {
+            // recycle the events of launches that have completed (oldest first)
+            for (int i = 0; i < 256 && !pending.empty(); ++i)
+                if (!resolve_front(false)) break;
+        }
+        if (pool.empty()) grow(64);
+        hipEvent_t e = pool.back();
+        pool.pop_back();
         return e;
     }
 };
@@ -71,16 +101,7 @@ void prof_push(int cls, hipEvent_t a, hipEvent_t b) {
 void prof_flush() {
     Profiler &p = P();
     std::lock_guard<std::mutex> lk(p.mu);
-    for (auto &q : p.pending) {
-        WUST_HIP_CHECK(hipEventSynchronize(q.b));
-        float ms = 0;
-        WUST_HIP_CHECK(hipEventElapsedTime(&ms, q.a, q.b));
-        p.total_ms[q.cls] += ms;
-        p.count[q.cls] += 1;
-        p.pool.push_back(q.a);
-        p.pool.push_back(q.b);
-    }
-    p.pending.clear();
+    while (!p.pending.empty()) p.resolve_front(true);
 }
 
 }  // namespace wust
@@ -88,7 +109,14 @@ void prof_flush() {
 extern "C" {
 
 int wust_profile_enable(uint32_t class_mask) {
-    wust::P().mask = class_mask;
+    wust::Profiler &p = wust::P();
+    try {
+        std::lock_guard<std::mutex> lk(p.mu);
+        if (class_mask && p.pool.size() < 4096) p.grow(4096 - (int)p.pool.size());
+    } catch (const wust::Error &e) {
+        return e.status;
+    }
+    p.mask = class_mask;
     return WUST_OK;
 }
 

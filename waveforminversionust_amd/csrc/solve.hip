@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(256) sweep_gemm_kernel(SweepParams p) {
     }
 }
 
-constexpr int SWEEP_RB = 2;   // 32-row tiles: 2 * 18 * 16 = 576 WGs at 560^2 x 256 RHS
+constexpr int SWEEP_RB = 1;   // 16-row tiles: 2 * 36 * 16 = 1152 WGs at 560^2 x 256 RHS
 
 }  // namespace
 
