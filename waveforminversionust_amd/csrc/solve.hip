@@ -83,8 +83,9 @@ __global__ void __launch_bounds__(256) sweep_operand_kernel(SweepParams p) {
 // products per complex product, two k-blocks of operands in flight, straight
 // from global memory in the MFMA lane layout); the partial tiles are summed
 // through LDS.  Tiles
-// are dealt to XCDs in contiguous runs (blockIdx % 8 is the XCD), so one
-// XCD's L2 serves a band of rows of X and a band of RHS columns.
+// are dealt to XCDs in contiguous runs (blockIdx % 8 is the XCD) with the RHS
+// block fastest, so one XCD reads a band of rows of X (1/4 of a page at two
+// chains on 8 XCDs) and the whole operand block, instead of whole pages.
 template <int RB, int ALAY>
 __global__ void __launch_bounds__(256) sweep_gemm_kernel(SweepParams p) {
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
@@ -92,8 +93,8 @@ __global__ void __launch_bounds__(256) sweep_gemm_kernel(SweepParams p) {
     if (L >= p.units) return;
     const int npad = p.npad, Rpad = p.Rpad;
     const int Mb = npad / (16 * RB), Nb = Rpad / 16;
-    const int mb = L % Mb;
-    const int nb = (L / Mb) % Nb;
+    const int nb = L % Nb;
+    const int mb = (L / Nb) % Mb;
     const int task = L / (Nb * Mb);
     const SweepTask &tk = p.task[task];
     const int i0 = mb * 16 * RB, j0 = nb * 16;
