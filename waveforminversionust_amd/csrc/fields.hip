@@ -89,76 +89,47 @@ __constant__ int c_dy[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
 __constant__ int c_dx[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
 __constant__ int c_opp[9] = {DC, DS, DN, DE, DSE, DNE, DW, DSW, DNW};
 
-// One thread per (RHS r, row y) marches along a chunk of RES_XC grid columns
-// keeping the 3 x 3 neighbourhood of u = u_old + d in registers (and the
-// next column in flight), so every field value is read from memory once per
-// chunk (plus the halo rows and columns) instead of once per neighbour.  Per-RHS norms are summed in
-// registers over the chunk and once per workgroup in LDS before the atomics.
-constexpr int RES_XC = 16;
-
+// One thread per (RHS r, row y, column x): the nine neighbours are
+// independent loads (more memory-level parallelism than a register window
+// marching along x, which measured slower), fused u += d, per-RHS |r|^2 and
+// |b|^2 reduced per workgroup in LDS before the atomics.
 __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
     const int r = blockIdx.x * 16 + (threadIdx.x & 15);
     const int yi = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int x0 = blockIdx.z * RES_XC;
+    const int xi = blockIdx.z;
     const int n = a.n, m = a.m, npad = a.npad, Rpad = a.Rpad;
-    const int x1 = (x0 + RES_XC < m) ? x0 + RES_XC : m;
-    const int64_t plane = (int64_t)m * n;
-    const bool active = yi < npad && r < Rpad;
-    const bool interior = active && yi < n;
-    const bool has_u = a.u_old || a.d32;
-    auto fetch = [&](int xx, int yy) {
-        cd u = mkd(0, 0);
-        if (!has_u || xx < 0 || xx >= m || yy < 0 || yy >= n) return u;
-        const int64_t on = ((int64_t)xx * npad + yy) * Rpad + r;
-        if (a.u_old) u = a.u_old[on];
-        if (a.d32) u = u + tod(a.d32[on]);
-        return u;
-    };
-    // win[cx][cy]: u at column x - 1 + cx, row yi - 1 + cy; nxt: column x + 2
-    // (requested one column ahead so its loads overlap this column's stencil)
-    cd win[3][3], nxt[3];
-    cd bnext = mkd(0, 0);
-    if (interior) {
-#pragma unroll
-        for (int cx = 0; cx < 3; ++cx)
-#pragma unroll
-            for (int cy = 0; cy < 3; ++cy) win[cx][cy] = fetch(x0 - 1 + cx, yi - 1 + cy);
-        if (a.b64 && x0 < x1) bnext = a.b64[((int64_t)x0 * npad + yi) * Rpad + r];
-    }
     double contrib = 0, bcontrib = 0;
-    for (int xi = x0; xi < x1; ++xi) {
+    if (yi < npad && r < Rpad) {
         const int64_t o = ((int64_t)xi * npad + yi) * Rpad + r;
-        if (interior) {
-            const cd b = bnext;
-#pragma unroll
-            for (int cy = 0; cy < 3; ++cy) nxt[cy] = fetch(xi + 2, yi - 1 + cy);
-            if (a.b64 && xi + 1 < x1) bnext = a.b64[o + (int64_t)npad * Rpad];
-            cd hu = mkd(0, 0);
-#pragma unroll
-            for (int dd = 0; dd < 9; ++dd) {
-                const int yy = yi + c_dy[dd], xx = xi + c_dx[dd];
-                if (yy < 0 || yy >= n || xx < 0 || xx >= m) continue;
-                cd w;
-                if (!a.adjoint) w = a.w64[dd * plane + (int64_t)xi * n + yi];
-                else w = conjd(a.w64[c_opp[dd] * plane + (int64_t)xx * n + yy]);
-                hu = hu + w * win[1 + c_dx[dd]][1 + c_dy[dd]];
-            }
-            const cd res = b - hu;
-            if (a.u_new) a.u_new[o] = win[1][1];
-            if (a.res32) a.res32[o] = tof(res);
-            if (a.res64) a.res64[o] = res;
-            contrib += abs2(res);
-            bcontrib += abs2(b);
-#pragma unroll
-            for (int cy = 0; cy < 3; ++cy) {
-                win[0][cy] = win[1][cy];
-                win[1][cy] = win[2][cy];
-                win[2][cy] = nxt[cy];
-            }
-        } else if (active) {
+        if (yi >= n) {
             if (a.u_new) a.u_new[o] = mkd(0, 0);
             if (a.res32) a.res32[o] = mk(0.f, 0.f);
             if (a.res64) a.res64[o] = mkd(0, 0);
+        } else {
+            const int64_t plane = (int64_t)m * n;
+            cd hu = mkd(0, 0);
+            cd self = mkd(0, 0);
+#pragma unroll
+            for (int dd = 0; dd < 9; ++dd) {
+                int yy = yi + c_dy[dd], xx = xi + c_dx[dd];
+                if (yy < 0 || yy >= n || xx < 0 || xx >= m) continue;
+                int64_t on = ((int64_t)xx * npad + yy) * Rpad + r;
+                cd u = mkd(0, 0);
+                if (a.u_old) u = a.u_old[on];
+                if (a.d32) u = u + tod(a.d32[on]);
+                if (dd == DC) self = u;
+                cd w;
+                if (!a.adjoint) w = a.w64[dd * plane + (int64_t)xi * n + yi];
+                else w = conjd(a.w64[c_opp[dd] * plane + (int64_t)xx * n + yy]);
+                hu = hu + w * u;
+            }
+            cd b = a.b64 ? a.b64[o] : mkd(0, 0);
+            cd res = b - hu;
+            if (a.u_new) a.u_new[o] = self;
+            if (a.res32) a.res32[o] = tof(res);
+            if (a.res64) a.res64[o] = res;
+            contrib = abs2(res);
+            bcontrib = abs2(b);
         }
     }
     if (a.norm2 || a.bnorm2) {
@@ -169,10 +140,10 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
         if (threadIdx.x < 32) {
             const int which = threadIdx.x >> 4, lane = threadIdx.x & 15;
             double *dst = which ? a.bnorm2 : a.norm2;
-            double sum = 0;
-            for (int k = 0; k < 16; ++k) sum += part[which][k][lane];
+            double s = 0;
+            for (int k = 0; k < 16; ++k) s += part[which][k][lane];
             const int rr = blockIdx.x * 16 + lane;
-            if (dst && rr < Rpad && sum != 0) atomicAdd(dst + rr, sum);
+            if (dst && rr < Rpad && s != 0) atomicAdd(dst + rr, s);
         }
     }
 }
@@ -382,7 +353,7 @@ void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int 
     WUST_LAUNCH_CHECK();
 }
 void launch_residual(const ResidualArgs &a, hipStream_t s) {
-    dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, (a.m + RES_XC - 1) / RES_XC);
+    dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);
     launch_k(PC_RESIDUAL, residual_kernel, grid, dim3(256), 0, s, a);
 }
 void promote(const cf *in, cd *out, int64_t count, hipStream_t s) {
