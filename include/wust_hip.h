@@ -185,6 +185,45 @@ int wust_directional_curvature(wust_helmholtz *h, const double *d_direction, con
  * last curvature consumer, Runtime/matlab/wust_reconstruct.m:61). */
 int wust_oracle_release(wust_helmholtz *h);
 
+/* ---- NCG step around the oracle (FP64, device arrays, [Ny, Nx] grids) ------------------ */
+/* ROI gradient and its filtered copy: roi = mask ? g : 0, z = mask ? F(roi) : 0,
+ * with F the radial Butterworth low-pass of cutoff * 2 f / c0 cycles/m and the
+ * given order (F = identity when cutoff == 0).  Grid spacing dx (along x,
+ * columns) and dy (along y, rows).  Replaces wust_roi_gradient
+ * (Runtime/matlab/wust_roi_gradient.m:1-10) and ringingRemovalFilt
+ * (Runtime/solver/ringingRemovalFilt.m:1-45). */
+int wust_roi_gradient(const double *d_g, const uint8_t *d_mask, int32_t ny, int32_t nx, double dx, double dy,
+                      double c0, double frequency_hz, double cutoff, double order, double *d_roi, double *d_z,
+                      void *stream);
+
+/* Clipped PR/FR direction with the objective's slope: gz = <g, z>; beta =
+ * clip(PR, 0, FR) on the same frequency (prev_gz > 0), else 0; direction =
+ * -z + beta * prev_direction, restarted as -z when <g, direction> >= 0.
+ * Outputs (host) beta, gz and slope = <g, direction>; WUST_NUMERICAL_FAILURE
+ * for a non-finite direction or gz < 0.  prev_z / prev_direction may be NULL
+ * when same_frequency == 0.  Replaces wust_ncg_direction
+ * (Runtime/matlab/wust_ncg_direction.m:1-17). */
+int wust_ncg_direction(const double *d_g, const double *d_z, const double *d_prev_z, double prev_gz,
+                       const double *d_prev_direction, int same_frequency, int64_t count, double *d_direction,
+                       double *beta, double *gz, double *slope, void *stream);
+
+/* alpha = max(0, -slope / curvature) (0 when curvature == 0 and slope == 0);
+ * WUST_NUMERICAL_FAILURE for non-finite input, negative curvature, or zero
+ * curvature with a nonzero slope.  Host only.  Replaces
+ * linearized_step_length (Runtime/matlab/linearized_step_length.m:1-11). */
+int wust_linearized_step_length(double slope, double curvature, double *alpha);
+
+/* Bounded slowness update: s = 1/c + step_damping * alpha * direction,
+ * clipped to [1/min(c + max_update, hi), 1/max(c - max_update, lo)], c_new =
+ * 1/s on the update mask and c elsewhere.  relative_update (host, may be
+ * NULL) = ||1/c_new - 1/c|| / ||1/c|| over the mask.  Replaces
+ * wust_project_slowness_update (Runtime/matlab/wust_project_slowness_update.m:1-10)
+ * and the record field of Runtime/matlab/wust_reconstruct.m:65-66. */
+int wust_project_slowness_update(const double *d_c, const double *d_direction, double alpha, double step_damping,
+                                 double max_update_mps, double bound_lo_mps, double bound_hi_mps,
+                                 const uint8_t *d_update_mask, int64_t count, double *d_c_new,
+                                 double *relative_update, void *stream);
+
 /* ---- tooling: per-kernel-class HIP-event timing (no reference counterpart) ------------ */
 /* Classes: 0 assembly, 1 Schur update, 2 Gauss-Jordan panel, 3 sweep operand,
  * 4 sweep GEMM, 5 FP64 residual, 6 source fit, 7 gradient, 8 plain factor step,

@@ -86,6 +86,16 @@ EXPORTS = {
     "wust_profile_enable": (C.c_int, [C.c_uint32]),
     "wust_profile_read": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "wust_profile_reset": (C.c_int, []),
+    "wust_roi_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]),
+    "wust_ncg_direction": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int,
+                                     C.c_int64, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.c_void_p]),
+    "wust_linearized_step_length": (C.c_int, [C.c_double, C.c_double, C.POINTER(C.c_double)]),
+    "wust_project_slowness_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                               C.c_double, C.c_double, C.c_void_p, C.c_int64, C.c_void_p,
+                                               C.POINTER(C.c_double), C.c_void_p]),
 }
 
 PROFILE_CLASSES = ["assembly", "schur", "gj_panel", "sweep_operand", "sweep_gemm", "residual", "source_fit",

@@ -9,6 +9,7 @@
 
 #include "engine.h"
 #include "fields.h"
+#include "ncg.h"
 #include "plain.h"
 #include "profile.h"
 
@@ -695,6 +696,71 @@ int wust_oracle_release(wust_helmholtz *h) {
     return guarded([&] {
         require(h != nullptr, "null handle");
         h->release_retained();
+    });
+}
+
+}  // extern "C"
+
+// ---- NCG step ----------------------------------------------------------------------
+
+extern "C" {
+
+int wust_roi_gradient(const double *d_g, const uint8_t *d_mask, int32_t ny, int32_t nx, double dx, double dy,
+                      double c0, double frequency_hz, double cutoff, double order, double *d_roi, double *d_z,
+                      void *stream) {
+    return guarded([&] {
+        require(d_g && d_mask && d_roi && d_z, "null argument");
+        require(ny >= 1 && nx >= 1, "Expected a nonempty [ny,nx] grid");
+        require(std::isfinite(cutoff) && cutoff >= 0, "cutoff must be a finite nonnegative scalar");
+        if (cutoff > 0) {
+            require(std::isfinite(dx) && std::isfinite(dy) && dx > 0 && dy > 0, "Uniform increasing grids required");
+            require(std::isfinite(c0) && c0 > 0 && std::isfinite(frequency_hz) && frequency_hz > 0,
+                    "c0 and frequency must be positive");
+            require(std::isfinite(order) && order > 0, "filter order must be positive");
+        }
+        ncg_roi_gradient(d_g, d_mask, ny, nx, dx, dy, c0, frequency_hz, cutoff, order, d_roi, d_z,
+                         (hipStream_t)stream);
+    });
+}
+
+int wust_ncg_direction(const double *d_g, const double *d_z, const double *d_prev_z, double prev_gz,
+                       const double *d_prev_direction, int same_frequency, int64_t count, double *d_direction,
+                       double *beta, double *gz, double *slope, void *stream) {
+    return guarded([&] {
+        require(d_g && d_z && d_direction && beta && gz && count > 0, "null or empty argument");
+        if (same_frequency && prev_gz > 0)
+            require(d_prev_z && d_prev_direction, "previous z and direction are required on the same frequency");
+        ncg_direction(d_g, d_z, d_prev_z, prev_gz, d_prev_direction, same_frequency != 0, count, d_direction, beta,
+                      gz, slope, (hipStream_t)stream);
+    });
+}
+
+int wust_linearized_step_length(double slope, double curvature, double *alpha) {
+    return guarded([&] {
+        require(alpha != nullptr, "null argument");
+        if (!(std::isfinite(slope) && std::isfinite(curvature) && curvature >= 0))
+            throw Error(WUST_NUMERICAL_FAILURE, "Nonfinite slope/curvature");
+        if (!(curvature > 0 || slope == 0))
+            throw Error(WUST_NUMERICAL_FAILURE, "Zero curvature with nonzero directional slope");
+        *alpha = (curvature > 0) ? std::max(0.0, -slope / curvature) : 0.0;
+    });
+}
+
+int wust_project_slowness_update(const double *d_c, const double *d_direction, double alpha, double step_damping,
+                                 double max_update_mps, double bound_lo_mps, double bound_hi_mps,
+                                 const uint8_t *d_update_mask, int64_t count, double *d_c_new,
+                                 double *relative_update, void *stream) {
+    return guarded([&] {
+        require(d_c && d_direction && d_update_mask && d_c_new && count > 0, "null or empty argument");
+        require(std::isfinite(bound_lo_mps) && std::isfinite(bound_hi_mps) && bound_lo_mps > 0 &&
+                    bound_hi_mps > bound_lo_mps,
+                "Invalid speed bounds");
+        require(std::isfinite(max_update_mps) && max_update_mps > 0 && std::isfinite(step_damping) &&
+                    step_damping > 0,
+                "Expected positive finite max_update_mps and step_damping");
+        require(std::isfinite(alpha) && alpha >= 0, "alpha must be finite and nonnegative");
+        ncg_project(d_c, d_direction, alpha, step_damping, max_update_mps, bound_lo_mps, bound_hi_mps,
+                    d_update_mask, count, d_c_new, relative_update, (hipStream_t)stream);
     });
 }
 
