@@ -1,0 +1,100 @@
+"""The reference's CPU-runnable case (160x160 = 128^2 interior + 16-cell PML, 64-element ring,
+two discs, 0.5 MHz, 40 dB noise) on the GPU engine, against the FP64 oracle.
+
+* Differentiation through ``WustOracleFunction`` with FP32 and FP64 slowness:
+  gradient vs the oracle, and central finite differences along 16 seeded
+  smooth directions (relative step 1e-4) -- the checks of
+  Runtime/tests/test_runtime.m:192-201, tightened.
+* Five NCG iterations of ``wust_reconstruct`` vs the oracle's loop
+  (initial 1500 m/s, update mask r < 26 mm, bounds 1400-1600 m/s).
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import make_obs
+from oracle import helmholtz_fp64 as H
+from oracle import ncg_fp64 as O
+from waveforminversionust_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+G = pytest.importorskip("waveforminversionust_amd.helmholtz")
+R = pytest.importorskip("waveforminversionust_amd.reconstruct")
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    case = W.cpu_reference_case()
+    obs = make_obs(case, snr_db=40.0, seed=5)
+    return case, obs
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def test_autograd_gradient_and_finite_differences(cfg1):
+    case, obs = cfg1
+    cfg = dict(case.cfg, tol=1e-12)
+    init = 1500 * np.ones((case.ny, case.nx))
+    lo, go = H.wust_oracle(init, obs, 0, case.cfg)
+    s = torch.full((case.ny, case.nx), 1 / 1500, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss = G.wust_loss(s, obs, 0, cfg)
+    (g,) = torch.autograd.grad(loss, s)
+    assert g.dtype == torch.float64 and g.shape == s.shape
+    assert abs(float(loss) - lo) / lo <= 1e-8 and _rel(g.cpu().numpy(), go) <= 1e-8
+    # FP32 slowness: the engine works from its FP64 value; the gradient comes back rounded to FP32
+    s32 = torch.full((case.ny, case.nx), 1 / 1500, dtype=torch.float32, device="cuda", requires_grad=True)
+    loss32 = G.wust_loss(s32, obs, 0, cfg)
+    (g32,) = torch.autograd.grad(loss32, s32)
+    l64, g64 = G.wust_oracle(1 / s32.detach().double(), obs, 0, cfg)
+    assert g32.dtype == torch.float32 and float(loss32) == pytest.approx(l64, rel=1e-12)
+    assert torch.allclose(g32, g64.float(), rtol=1e-6, atol=0)
+    assert _rel(g32.double().cpu().numpy(), go) <= 1e-4
+    s0 = torch.full((case.ny, case.nx), 1 / 1500, dtype=torch.float64, device="cuda")
+    g = torch.as_tensor(go, device="cuda")
+    rng = np.random.default_rng(2024)
+    h = 1e-4
+    worst = 0.0
+    for _ in range(16):
+        d = W.gaussian_filter2(rng.standard_normal((case.ny, case.nx)), 4.0)
+        d = torch.as_tensor(d / np.max(np.abs(d)), device="cuda") * s0          # relative direction
+        fp = float(G.wust_loss(s0 + h * d, obs, 0, cfg))
+        fm = float(G.wust_loss(s0 - h * d, obs, 0, cfg))
+        fd = (fp - fm) / (2 * h)
+        an = float(torch.sum(g * d))
+        worst = max(worst, abs(fd - an) / abs(an))
+    assert worst <= 1e-6, worst
+
+
+def test_backward_scales_and_places_gradient(cfg1):
+    case, obs = cfg1
+    s = torch.full((case.ny, case.nx), 1 / 1500, dtype=torch.float32, requires_grad=True)   # host tensor
+    loss = G.wust_loss(s, obs, 0, case.cfg)
+    (3.0 * loss).backward()
+    _, g = G.wust_oracle(1 / s.detach().double(), obs, 0, case.cfg)
+    assert s.grad.device.type == "cpu" and s.grad.dtype == torch.float32
+    assert torch.allclose(s.grad.double(), 3.0 * g.cpu().double(), rtol=1e-6, atol=0)
+    with pytest.raises(G.WustError):
+        G.wust_loss(-s.detach(), obs, 0, case.cfg)
+
+
+def test_five_ncg_iterations_match_oracle(cfg1):
+    case, obs = cfg1
+    X, Y = np.meshgrid(case.x_m, case.y_m)
+    mask = X ** 2 + Y ** 2 < 26e-3 ** 2
+    cfg = dict(case.cfg, step_damping=0.25, max_update_mps=12.0, bounds_mps=[1400.0, 1600.0], filter_cutoff=0.0,
+               filter_order=4, update_mask=mask, frequency_schedule_hz=[0.5e6] * 5)
+    init = 1500 * np.ones((case.ny, case.nx))
+    res = R.wust_reconstruct(obs, init, cfg)
+    c, rec = O.wust_reconstruct(obs, init, dict(cfg, schedule=[0] * 5), H.wust_oracle, H.directional_curvature)
+    for got, want in zip(res["records"], rec):
+        assert got["loss_before_update"] == pytest.approx(want["loss_before_update"], rel=1e-8)
+        assert got["alpha"] == pytest.approx(want["alpha"], rel=1e-6)
+        assert got["beta"] == pytest.approx(want["beta"], rel=1e-5, abs=1e-8)
+    assert np.max(np.abs(res["c_mps"].cpu().numpy() - c)) <= 1e-5
+    assert rec[-1]["loss_before_update"] < 0.5 * rec[0]["loss_before_update"]
+    err0 = np.linalg.norm((init - case.c_true)[mask])
+    err = np.linalg.norm((res["c_mps"].cpu().numpy() - case.c_true)[mask])
+    assert err < err0

@@ -8,6 +8,8 @@ Same names, argument meaning and error identifiers as the reference:
 * :func:`wust_oracle` -- loss / gradient / retained state of one frequency
   (Runtime/matlab/wust_oracle.m:1-86), ``nargout`` selecting the work.
 * :func:`directional_curvature` (Runtime/matlab/directional_curvature.m).
+* :class:`WustOracleFunction` / :func:`wust_loss` -- the oracle as a torch
+  ``autograd.Function`` of slowness (loss forward, gradient backward).
 
 Arrays follow the reference's MATLAB conventions: grids are ``[y, x]``,
 fields ``[y, x, source]``, node indices 1-based ``y + Ny*x + 1``.  Inputs may
@@ -323,6 +325,39 @@ def directional_curvature(state, direction, rx_index=None, receiver_mask=None):
             solver._handle(), C.c_void_p(d.data_ptr()), C.c_void_p(Md.data_ptr()), float(solver.tol),
             int(solver.max_iter), C.byref(kappa), _stream(dev)))
     return kappa.value
+
+
+class WustOracleFunction(torch.autograd.Function):
+    """Misfit of one frequency as a differentiable function of slowness.
+
+    ``forward(ctx, slowness, obs, fi, cfg)`` evaluates :func:`wust_oracle` at
+    c = 1 / slowness (loss and gradient in one engine call: forward solves,
+    FP64 source fit, adjoint solves reusing the factors, stencil-derivative
+    gradient) and returns the FP64 loss as a 0-d tensor; ``backward`` returns
+    ``grad_output * g`` in the dtype and on the device of ``slowness``.  The
+    slowness may be FP32 or FP64; the engine always works from its FP64 value.
+    """
+
+    @staticmethod
+    def forward(ctx, slowness, obs, fi, cfg):
+        s = slowness.detach()
+        if s.dim() != 2 or not bool(torch.all(torch.isfinite(s) & (s > 0))):
+            raise WustError("WUST:InvalidInput", "Expected a positive finite [ny,nx] slowness")
+        loss, g = wust_oracle(1.0 / s.to(torch.float64), obs, fi, cfg, nargout=2)
+        ctx.save_for_backward(g)
+        ctx.in_dtype, ctx.in_device = slowness.dtype, slowness.device
+        return torch.tensor(loss, dtype=torch.float64, device=slowness.device)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        (g,) = ctx.saved_tensors
+        grad = grad_output.to(device=g.device, dtype=torch.float64) * g
+        return grad.to(device=ctx.in_device, dtype=ctx.in_dtype), None, None, None
+
+
+def wust_loss(slowness, obs, fi, cfg):
+    """``WustOracleFunction.apply``: differentiable misfit Phi_f(s) of frequency ``fi``."""
+    return WustOracleFunction.apply(slowness, obs, fi, cfg)
 
 
 def stencil_opt_params(vmin, vmax, f, h, g):

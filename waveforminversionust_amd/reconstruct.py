@@ -278,7 +278,8 @@ def _frequency_data_norm(obs):
         return np.asarray(obs["frequency_data_norm"], np.float64).ravel()
     Y = obs["Y"]
     Y = Y.detach().cpu().numpy() if torch.is_tensor(Y) else np.asarray(Y)
-    return np.sum(np.abs(Y.astype(np.complex128)) ** 2, axis=(0, 1))
+    with np.errstate(over="ignore"):       # huge data gives inf, as the reference's sum does
+        return np.sum(np.abs(Y.astype(np.complex128)) ** 2, axis=(0, 1))
 
 
 def wust_reconstruct(obs, initial, cfg):
