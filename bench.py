@@ -64,7 +64,208 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 oracle accuracy / CPU baseline leg")
     ap.add_argument("--cpu-tx", type=int, default=8, help="transmitters in the CPU/accuracy sample")
+    ap.add_argument("--workload", default="ring560", choices=["ring560", "continuation8", "breast1088", "dtft"],
+                    help="ring560: the metric case (default); continuation8: eight frequency stages on their own "
+                         "grids, one grad eval each; breast1088: a full 36-step reconstruction at 1088^2 x 512 TX; "
+                         "dtft: RF -> frequency DTFT of 256 TX x 256 RX x 4096 float32 samples, 12 frequencies")
+    ap.add_argument("--concurrent", action="store_true",
+                    help="continuation8: run the stages on separate streams from host threads")
     return ap.parse_args()
+
+
+def synthetic_obs(G, case, dev, tol):
+    """Refined forward solves on the true model at every frequency of the case, gathered at the receivers."""
+    import torch
+    ny, nx, ntx = case.ny, case.nx, case.n_tx
+    Y = torch.zeros((ntx, case.n_rx, len(case.frequencies_hz)), dtype=torch.complex128, device=dev)
+    tx0 = torch.as_tensor(case.tx_index - 1, device=dev)
+    rx0 = torch.as_tensor(case.rx_index - 1, device=dev)
+    opts = dict(backend="gpu", stencil_bounds=case.cfg["stencil_bounds"], tol=tol, device=dev)
+    for fi, f in enumerate(case.frequencies_hz):
+        truth = G.HelmholtzSolver(case.x_m, case.y_m, torch.as_tensor(case.c_true, device=dev), None, f, -1,
+                                  case.cfg["pml_strength"], case.cfg["pml_m"], opts)
+        src = torch.zeros((ntx, nx, ny), dtype=torch.complex128, device=dev)
+        src.view(ntx, -1)[torch.arange(ntx, device=dev), tx0] = 1
+        u, _ = truth.solve(src.permute(2, 1, 0), False)
+        Y[:, :, fi] = u.permute(2, 1, 0).reshape(ntx, -1)[:, rx0]
+        truth.close()
+        del u, src
+    return dict(x_m=case.x_m, y_m=case.y_m, shape_yx=(ny, nx), n_tx=ntx, tx_index=case.tx_index,
+                rx_index=case.rx_index, frequencies_hz=list(case.frequencies_hz), Y=Y,
+                mask=torch.ones((ntx, case.n_rx, len(case.frequencies_hz)), dtype=torch.bool, device=dev))
+
+
+def run_continuation(args):
+    """Eight frequency stages on their own grids (201^2 ... 669^2 interiors + 24-cell PML, 256 TX each);
+    one pass = one refined grad eval per stage at homogeneous 1500 m/s."""
+    import threading
+
+    import torch
+
+    from waveforminversionust_amd import helmholtz as G
+    from waveforminversionust_amd import workloads as W
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    cases = W.continuation_cases()
+    stages = []
+    for case in cases:
+        obs = synthetic_obs(G, case, dev, args.tol)
+        c0 = torch.full((case.ny, case.nx), 1500.0, dtype=torch.float64, device=dev)
+        stages.append((case, obs, c0, dict(case.cfg, mode="refined", tol=args.tol, device=dev)))
+    log("stages: " + ", ".join(f"{c.ny}^2 @ {c.frequencies_hz[0] / 1e6:.1f} MHz" for c in cases))
+    streams = [torch.cuda.Stream(device=dev) for _ in stages] if args.concurrent else None
+    per_stage = [0.0] * len(stages)
+
+    def one(i):
+        case, obs, c0, cfg = stages[i]
+        t = time.perf_counter()
+        if streams:
+            with torch.cuda.stream(streams[i]):
+                G.wust_oracle(c0, obs, 0, cfg)
+        else:
+            G.wust_oracle(c0, obs, 0, cfg)
+        per_stage[i] += time.perf_counter() - t
+
+    def one_pass():
+        if streams:
+            th = [threading.Thread(target=one, args=(i,)) for i in range(len(stages))]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+        else:
+            for i in range(len(stages)):
+                one(i)
+
+    for _ in range(args.warmup):
+        one_pass()
+    torch.cuda.synchronize()
+    per_stage = [0.0] * len(stages)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_pass()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    line = {
+        "metric": "continuation passes/s (8 stages x 256 TX, one refined grad eval per stage)",
+        "value": round(args.steps / elapsed, 4), "unit": "passes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 (fp32 MFMA) + f64 refinement", "data": "synthetic",
+        "config": {"workload": "continuation8", "concurrent_streams": bool(args.concurrent),
+                   "stages": [dict(grid=c.ny, frequency_hz=c.frequencies_hz[0], dx_m=c.dx,
+                                   ms=round(1e3 * per_stage[i] / args.steps, 2)) for i, c in enumerate(cases)],
+                   "refine_tol": args.tol},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_dtft(args):
+    """RF -> frequency DTFT: 256 TX x 256 RX x 4096 float32 samples (40 MHz), 12 frequencies 0.3-1.2 MHz."""
+    import numpy as np
+    import torch
+
+    from waveforminversionust_amd import observations as OB
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    n_time, n_rx, n_tx, dt = 4096, 256, 256, 25e-9
+    freqs = np.linspace(0.3e6, 1.2e6, 12)
+    t = 1e-6 + np.arange(n_time) * dt
+    g = torch.Generator(device=dev).manual_seed(5)
+    rf = torch.randn((n_tx, n_rx, n_time), generator=g, device=dev, dtype=torch.float32).permute(2, 1, 0)
+    active = np.ones((n_tx, n_rx), bool)
+    for _ in range(args.warmup):
+        OB.rf_dtft(rf, t, freqs, active)
+    torch.cuda.synchronize()
+    kms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        P, ms = OB.rf_dtft(rf, t, freqs, active, return_kernel_ms=True)
+        kms += ms
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ncols = n_rx * n_tx
+    rf_bytes = n_time * ncols * 4
+    out_bytes = ncols * len(freqs) * 8
+    flops = 2.0 * 2 * len(freqs) * n_time * ncols
+    k_s = kms / 1e3 / args.steps
+    gbs = (rf_bytes + out_bytes) / k_s / 1e9
+    tfs = flops / k_s / 1e12
+    cpu = None
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+
+        from oracle import observations_fp64 as O
+        sub = 8
+        x = rf[:, :, :sub].cpu().numpy()
+        with threadpool_limits(1):
+            c0 = time.perf_counter()
+            want = O.rf_dtft(x, t, freqs, active[:sub], single=False)
+            c_s = time.perf_counter() - c0
+        got = P[:sub].cpu().numpy().astype(np.complex128)
+        cpu = dict(value=round(sub / c_s, 3), unit="TX/s", cores=1, kind="port",
+                   sample=f"FP64 numpy oracle (1 thread) on {sub} of {n_tx} TX ({n_rx} RX x {n_time} samples): "
+                          f"{c_s:.3f} s; GPU rel err vs it {np.linalg.norm(got - want) / np.linalg.norm(want):.2e}")
+    line = {
+        "metric": "RF->frequency DTFT transmitters/s (256 RX x 4096 float32 samples, 12 frequencies)",
+        "value": round(n_tx * args.steps / elapsed, 1), "unit": "TX/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 (MFMA) from f32 RF, c64 result", "data": "synthetic",
+        "config": {"workload": "dtft_256tx_256rx_4096t_12f", "n_tx": n_tx, "n_rx": n_rx, "n_time": n_time,
+                   "n_freq": len(freqs)},
+        "roofline": {"kernel": "dtft_kernel", "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_us": round(k_s * 1e6, 2), "algorithmic_bytes_per_launch": rf_bytes + out_bytes,
+                     "fp64_mfma_tflops": round(tfs, 2), "fp64_mfma_frac": round(tfs / 78.6, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_breast(args):
+    """Full reconstruction: 12 frequencies 0.3-1.2 MHz x 3 NCG iterations on 1088^2 nodes, 512 TX."""
+    import numpy as np
+    import torch
+
+    from waveforminversionust_amd import helmholtz as G
+    from waveforminversionust_amd import reconstruct as Rc
+    from waveforminversionust_amd import workloads as W
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    case = W.breast_case()
+    t0 = time.perf_counter()
+    obs = synthetic_obs(G, case, dev, args.tol)
+    log(f"observations: {len(case.frequencies_hz)} forward solves in {time.perf_counter() - t0:.1f} s")
+    X, Y = np.meshgrid(case.x_m, case.y_m)
+    mask = X ** 2 + Y ** 2 < 55e-3 ** 2
+    cfg = dict(case.cfg, step_damping=0.25, max_update_mps=12.0, bounds_mps=[1400.0, 1650.0], filter_cutoff=0.5,
+               filter_order=4, update_mask=mask,
+               frequency_schedule_hz=[f for f in case.frequencies_hz for _ in range(3)], tol=args.tol, device=dev)
+    init = torch.full((case.ny, case.nx), 1500.0, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = Rc.wust_reconstruct(obs, init, cfg)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    rec = res["records"]
+    c = res["c_mps"].cpu().numpy()
+    err0 = float(np.linalg.norm((1500.0 - case.c_true)[mask]) / np.linalg.norm(case.c_true[mask]))
+    err = float(np.linalg.norm((c - case.c_true)[mask]) / np.linalg.norm(case.c_true[mask]))
+    line = {
+        "metric": "full reconstruction wall time (1088^2, 512 TX, 12 frequencies x 3 NCG steps)",
+        "value": round(elapsed, 2), "unit": "s", "n_gpus": 1, "steps": len(rec), "warmup": 0,
+        "ms_per_step": round(1e3 * elapsed / max(len(rec), 1), 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (fp32 MFMA) + f64 refinement", "data": "synthetic",
+        "config": {"workload": case.name, "grid": f"{case.ny}x{case.nx} (1024^2 interior + 32-cell PML)",
+                   "dx_m": case.dx, "n_tx": case.n_tx, "source_batch_size": case.cfg["source_batch_size"],
+                   "refine_tol": args.tol},
+        "per_step_ms": {"factor": round(1e3 * float(np.mean([r["factor_seconds"] for r in rec])), 1),
+                        "forward": round(1e3 * float(np.mean([r["forward_seconds"] for r in rec])), 1),
+                        "linearized": round(1e3 * float(np.mean([r["linearized_seconds"] for r in rec])), 1),
+                        "wall": round(1e3 * float(np.mean([r["wall_seconds"] for r in rec])), 1)},
+        "loss_first_last": [rec[0]["loss_before_update"], rec[-1]["loss_before_update"]],
+        "model_rel_err_initial_final": [round(err0, 6), round(err, 6)],
+    }
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -73,6 +274,12 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the transmitter-sharded multi-GPU layout is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "continuation8":
+        return run_continuation(args)
+    if args.workload == "breast1088":
+        return run_breast(args)
+    if args.workload == "dtft":
+        return run_dtft(args)
     import numpy as np
     import torch
 

@@ -224,6 +224,18 @@ int wust_project_slowness_update(const double *d_c, const double *d_direction, d
                                  const uint8_t *d_update_mask, int64_t count, double *d_c_new,
                                  double *relative_update, void *stream);
 
+/* ---- RF -> frequency DTFT (data preparation) ------------------------------------------- */
+/* pressure(tx, r, f) = single( sum_t exp(-2 pi i f_hz[f] time_s[t]) dt rf(t, r, tx) ) in FP64,
+ * with every sample of a channel whose active[tx + n_tx r] == 0 treated as zero.  rf: device
+ * [n_time, n_rx, n_tx] column-major, float32 (rf_dtype 0) or float64 (rf_dtype 1); time_s: device
+ * [n_time]; frequencies_hz: host [n_freq], each in (0, 0.5/dt); active: device uint8 [n_tx, n_rx];
+ * pressure: device complex64 [n_tx, n_rx, n_freq].  kernel_ms (host, may be NULL) receives the GEMM
+ * launches' duration.  WUST_INVALID_INPUT for a non-finite sample on an active channel.  Replaces the
+ * DTFT loop of wust_prepare (Runtime/matlab/wust_prepare.m:29-37). */
+int wust_rf_dtft(const void *d_rf, int rf_dtype, int64_t n_time, int32_t n_rx, int32_t n_tx, const double *d_time_s,
+                 double dt, const double *frequencies_hz, int32_t n_freq, const uint8_t *d_active, void *d_pressure,
+                 float *kernel_ms, void *stream);
+
 /* ---- tooling: per-kernel-class HIP-event timing (no reference counterpart) ------------ */
 /* Classes: 0 assembly, 1 Schur update, 2 Gauss-Jordan panel, 3 sweep operand,
  * 4 sweep GEMM, 5 FP64 residual, 6 source fit, 7 gradient, 8 plain factor step,

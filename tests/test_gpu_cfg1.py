@@ -43,29 +43,36 @@ def test_autograd_gradient_and_finite_differences(cfg1):
     loss = G.wust_loss(s, obs, 0, cfg)
     (g,) = torch.autograd.grad(loss, s)
     assert g.dtype == torch.float64 and g.shape == s.shape
-    assert abs(float(loss) - lo) / lo <= 1e-8 and _rel(g.cpu().numpy(), go) <= 1e-8
+    assert abs(loss.item() - lo) / lo <= 1e-8 and _rel(g.cpu().numpy(), go) <= 1e-8
     # FP32 slowness: the engine works from its FP64 value; the gradient comes back rounded to FP32
     s32 = torch.full((case.ny, case.nx), 1 / 1500, dtype=torch.float32, device="cuda", requires_grad=True)
     loss32 = G.wust_loss(s32, obs, 0, cfg)
     (g32,) = torch.autograd.grad(loss32, s32)
     l64, g64 = G.wust_oracle(1 / s32.detach().double(), obs, 0, cfg)
-    assert g32.dtype == torch.float32 and float(loss32) == pytest.approx(l64, rel=1e-12)
+    assert g32.dtype == torch.float32 and loss32.item() == pytest.approx(l64, rel=1e-12)
     assert torch.allclose(g32, g64.float(), rtol=1e-6, atol=0)
     assert _rel(g32.double().cpu().numpy(), go) <= 1e-4
     s0 = torch.full((case.ny, case.nx), 1 / 1500, dtype=torch.float64, device="cuda")
     g = torch.as_tensor(go, device="cuda")
     rng = np.random.default_rng(2024)
     h = 1e-4
-    worst = 0.0
+    worst, worst_rich = 0.0, 0.0
+
+    def fd(d, step):
+        return (float(G.wust_loss(s0 + step * d, obs, 0, cfg)) - float(G.wust_loss(s0 - step * d, obs, 0, cfg))) \
+            / (2 * step)
+
     for _ in range(16):
         d = W.gaussian_filter2(rng.standard_normal((case.ny, case.nx)), 4.0)
         d = torch.as_tensor(d / np.max(np.abs(d)), device="cuda") * s0          # relative direction
-        fp = float(G.wust_loss(s0 + h * d, obs, 0, cfg))
-        fm = float(G.wust_loss(s0 - h * d, obs, 0, cfg))
-        fd = (fp - fm) / (2 * h)
         an = float(torch.sum(g * d))
-        worst = max(worst, abs(fd - an) / abs(an))
-    assert worst <= 1e-6, worst
+        f1, f2 = fd(d, h), fd(d, h / 2)
+        worst = max(worst, abs(f1 - an) / abs(an))
+        worst_rich = max(worst_rich, abs((4 * f2 - f1) / 3 - an) / abs(an))
+    # the central difference carries its O(h^2) truncation (up to 5.6e-6 on these directions, the same
+    # with the FP64 oracle's own losses); the Richardson combination removes it
+    assert worst <= 1e-5, worst
+    assert worst_rich <= 1e-7, worst_rich
 
 
 def test_backward_scales_and_places_gradient(cfg1):
@@ -75,7 +82,8 @@ def test_backward_scales_and_places_gradient(cfg1):
     (3.0 * loss).backward()
     _, g = G.wust_oracle(1 / s.detach().double(), obs, 0, case.cfg)
     assert s.grad.device.type == "cpu" and s.grad.dtype == torch.float32
-    assert torch.allclose(s.grad.double(), 3.0 * g.cpu().double(), rtol=1e-6, atol=0)
+    # far from the array the slowness gradient falls below FLT_MIN (subnormal in FP32): compare at that floor
+    assert torch.allclose(s.grad.double(), 3.0 * g.cpu().double(), rtol=1e-6, atol=torch.finfo(torch.float32).tiny)
     with pytest.raises(G.WustError):
         G.wust_loss(-s.detach(), obs, 0, case.cfg)
 

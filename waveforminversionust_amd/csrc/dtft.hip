@@ -1,0 +1,207 @@
+// RF -> frequency DTFT of the data-preparation side (Runtime/matlab/wust_prepare.m:29-37):
+//
+//   P(tx, r, f) = single( sum_t exp(-2 pi i f t_s[t]) dt x(t, r, tx) ),
+//
+// with channels whose mask is false for every frequency zeroed before the sum.
+// The reference forms the [nf x Nt] kernel in FP64 and multiplies it with the
+// [Nt x nr] traces of one transmitter in FP64, then rounds to single.  Here
+// the whole acquisition is one real FP64 GEMM on MFMA,
+//
+//   C[rows x N] = K[rows x Nt] . X[Nt x N],  N = nr * nt columns (t fastest in X),
+//
+// where K holds the real and imaginary parts of the kernel as separate rows:
+// row tile T (16 rows) carries frequencies 8T..8T+7, rows 0-7 their real
+// parts and rows 8-15 their imaginary parts.  v_mfma_f64_16x16x4f64 takes
+// A[i][k] from lane i + 16 k and B[k][j] from lane j + 16 k, and leaves
+// D[i][j] in lane j + 16 (i % 4), register i / 4 (measured with one-hot
+// operands on gfx950).  Lane j + 16 g therefore holds rows g, g + 4, g + 8,
+// g + 12: the real and imaginary parts of frequencies 8T + g and 8T + g + 4
+// in registers (0, 2) and (1, 3), ready for complex64 stores.
+//
+// X is read once from HBM (the kernel is HBM-bound for double input and
+// balanced with FP64 MFMA for single input at ~12 frequencies); K is a small
+// L2-resident table staged through LDS one sample chunk at a time and shared
+// by the four waves of a workgroup.
+#include "dtft.h"
+
+namespace wust {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int DT_WAVES = 4;        // waves per workgroup, 16 columns each
+constexpr int DT_KC = 64;          // samples per LDS chunk (4 k-blocks of 16)
+constexpr int DT_LDS_PAD = 2;      // doubles of padding per LDS row
+
+// K[row][t] for row tiles of 8 frequencies (re rows 0-7, im rows 8-15), zero past
+// n_time and past n_freq.  theta = (-2 pi f) t in that order of rounding, as the
+// reference's exp(-2i*pi*frequency_hz*time_s.') evaluates it.
+__global__ void dtft_table_kernel(const double *time_s, const double *freqs, int n_freq, int64_t n_time,
+                                  int64_t nt_pad, int rows, double dt, double *K) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rows * nt_pad) return;
+    const int row = (int)(i / nt_pad);
+    const int64_t t = i - (int64_t)row * nt_pad;
+    const int tile = row >> 4, in = row & 15;
+    const int f = tile * 8 + (in & 7);
+    double v = 0.0;
+    if (f < n_freq && t < n_time) {
+        const double w = -2.0 * M_PI * freqs[f];
+        const double th = w * time_s[t];
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        v = (in < 8 ? cs : sn) * dt;
+    }
+    K[i] = v;
+}
+
+template <typename T>
+__device__ __forceinline__ double to_d(T v) {
+    return (double)v;
+}
+
+// One wave: 16 columns x RT row tiles over all samples.  Workgroup = DT_WAVES waves.
+template <int RT, typename T>
+__global__ void __launch_bounds__(64 * DT_WAVES) dtft_kernel(DtftParams p) {
+    __shared__ double ksh[RT * 16][DT_KC + DT_LDS_PAD];
+    const T *x = static_cast<const T *>(p.rf);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, kq = lane >> 4;
+    const int64_t col = ((int64_t)blockIdx.x * DT_WAVES + wave) * 16 + c;
+    const bool col_ok = col < p.ncols;
+    const int64_t colc = col_ok ? col : 0;
+    const int tx = (int)(colc / p.n_rx), rx = (int)(colc - (int64_t)tx * p.n_rx);
+    const bool active = col_ok && p.active[tx + (int64_t)p.n_tx * rx] != 0;
+    const T *xc = x + colc * p.n_time;
+    v4d acc[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) acc[r] = v4d{0.0, 0.0, 0.0, 0.0};
+    bool bad = false;
+    for (int64_t t0 = 0; t0 < p.nt_pad; t0 += DT_KC) {
+        // this lane's samples of the chunk: t0 + 16 b + 4 kq + s, b = k-block, s = MFMA step
+        double xv[DT_KC / 16][4];
+#pragma unroll
+        for (int b = 0; b < DT_KC / 16; ++b)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int64_t t = t0 + 16 * b + 4 * kq + s;
+                const bool in = active && t < p.n_time;
+                const double v = to_d(xc[in ? t : 0]);
+                if (in && !isfinite(v)) bad = true;
+                xv[b][s] = in ? v : 0.0;
+            }
+        __syncthreads();   // previous chunk's table reads are done
+        for (int e = threadIdx.x; e < RT * 16 * DT_KC; e += 64 * DT_WAVES) {
+            const int row = e / DT_KC, tt = e - row * DT_KC;
+            ksh[row][tt] = p.K[(int64_t)row * p.nt_pad + t0 + tt];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < DT_KC / 16; ++b)
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int r = 0; r < RT; ++r)
+                    acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ksh[16 * r + c][16 * b + 4 * kq + s], xv[b][s],
+                                                                  acc[r], 0, 0, 0);
+    }
+    if (__any(bad) && lane == 0) atomicOr(p.status, 1u);
+    // registers (q, q + 2) of lane c + 16 kq: real and imaginary parts of frequency 8 r + kq + 4 q
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int f = 8 * r + kq + 4 * q;
+            if (col_ok && f < p.n_freq_chunk) {
+                float2 v = make_float2((float)acc[r][q], (float)acc[r][q + 2]);
+                reinterpret_cast<float2 *>(p.out)[tx + (int64_t)p.n_tx * (rx + (int64_t)p.n_rx * (p.f0 + f))] = v;
+            }
+        }
+}
+
+template <typename T>
+void launch_rt(int rt, const DtftParams &p, dim3 grid, hipStream_t s) {
+    switch (rt) {
+        case 1: hipLaunchKernelGGL((dtft_kernel<1, T>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
+        case 2: hipLaunchKernelGGL((dtft_kernel<2, T>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
+        case 3: hipLaunchKernelGGL((dtft_kernel<3, T>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
+        default: hipLaunchKernelGGL((dtft_kernel<4, T>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
+    }
+    WUST_LAUNCH_CHECK();
+}
+
+template <typename T>
+struct Buf {
+    T *p = nullptr;
+    explicit Buf(int64_t n) { WUST_HIP_CHECK(hipMalloc(&p, sizeof(T) * (size_t)(n > 0 ? n : 1))); }
+    ~Buf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+void rf_dtft(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx, const double *time_s, double dt,
+             const double *freqs_host, int n_freq, const uint8_t *active, void *out, hipStream_t s,
+             float *kernel_ms) {
+    const int64_t nt_pad = (n_time + DT_KC - 1) / DT_KC * DT_KC;
+    const int64_t ncols = (int64_t)n_rx * n_tx;
+    Buf<double> freqs(n_freq);
+    Buf<uint32_t> status(1);
+    WUST_HIP_CHECK(hipMemcpyAsync(freqs.p, freqs_host, sizeof(double) * n_freq, hipMemcpyHostToDevice, s));
+    WUST_HIP_CHECK(hipMemsetAsync(status.p, 0, sizeof(uint32_t), s));
+    const dim3 grid((unsigned)((ncols + 16 * DT_WAVES - 1) / (16 * DT_WAVES)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (kernel_ms) {
+        WUST_HIP_CHECK(hipEventCreate(&e0));
+        WUST_HIP_CHECK(hipEventCreate(&e1));
+    }
+    float total_ms = 0.f;
+    // frequencies in chunks of 32 (four row tiles); each chunk is one pass over the RF
+    for (int f0 = 0; f0 < n_freq; f0 += 32) {
+        const int nf = n_freq - f0 < 32 ? n_freq - f0 : 32;
+        const int rt = (nf + 7) / 8;
+        const int rows = rt * 16;
+        Buf<double> K((int64_t)rows * nt_pad);
+        const int64_t nk = (int64_t)rows * nt_pad;
+        hipLaunchKernelGGL(dtft_table_kernel, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, time_s,
+                           freqs.p + f0, nf, n_time, nt_pad, rows, dt, K.p);
+        WUST_LAUNCH_CHECK();
+        DtftParams p{};
+        p.rf = rf;
+        p.K = K.p;
+        p.active = active;
+        p.out = out;
+        p.status = status.p;
+        p.n_time = n_time;
+        p.nt_pad = nt_pad;
+        p.ncols = ncols;
+        p.n_rx = n_rx;
+        p.n_tx = n_tx;
+        p.f0 = f0;
+        p.n_freq_chunk = nf;
+        if (kernel_ms) WUST_HIP_CHECK(hipEventRecord(e0, s));
+        if (is_double) launch_rt<double>(rt, p, grid, s);
+        else launch_rt<float>(rt, p, grid, s);
+        if (kernel_ms) {
+            WUST_HIP_CHECK(hipEventRecord(e1, s));
+            WUST_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            WUST_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            total_ms += ms;
+        }
+        WUST_HIP_CHECK(hipStreamSynchronize(s));   // K is freed at the end of the chunk
+    }
+    uint32_t st = 0;
+    WUST_HIP_CHECK(hipMemcpyAsync(&st, status.p, sizeof(st), hipMemcpyDeviceToHost, s));
+    WUST_HIP_CHECK(hipStreamSynchronize(s));
+    if (kernel_ms) {
+        *kernel_ms = total_ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    if (st) throw Error(WUST_INVALID_INPUT, "Nonfinite RF on an active channel");
+}
+
+}  // namespace wust

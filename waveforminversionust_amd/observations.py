@@ -9,6 +9,11 @@ and ``WUST:*`` identifiers:
   -- declared axes, Fourier sign, frequency order, masks and the derived
   per-frequency counts and energies of the canonical ``wust.measurements``;
 * :func:`wust_assert_schema` (Runtime/matlab/wust_assert_schema.m:1-5);
+* :func:`wust_prepare` (Runtime/matlab/wust_prepare.m:1-83) -- RF [time, rx, tx]
+  to canonical complex pressure: the dt-weighted DTFT runs on the GPU as one
+  FP64 MFMA GEMM over all transmitters (``wust_rf_dtft``), followed by the
+  ingestion above and the optional homogeneous time-of-flight phase
+  correction for the snapping offsets;
 * :func:`observations_to_device` -- the canonical observations placed once in
   HBM so that every oracle call reads its frequency slice in place: ``Y`` is
   stored as complex128 ``[frequency][rx][tx]`` (the column-major
@@ -21,9 +26,12 @@ double).
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import torch
 
+from . import _native as N
 from ._native import WustError
 
 _UNITS = ("Pa*s", "instrument_units", "unknown")
@@ -189,3 +197,114 @@ def observations_to_device(obs, device=None):
     out = dict(obs)
     out.update(Y=Yd, mask=Md)
     return out
+
+
+_RF_CFG = {"grid", "frequencies_hz", "c_geom_mps", "phase_correction", "mask", "data_units",
+           "measurement_provenance"}
+
+
+def rf_dtft(pressure, time_s, frequencies_hz, active, device=None, return_kernel_ms=False):
+    """Device DTFT: complex64 [tx, rx, frequency] from real RF [time, rx, tx] (float32 or float64).
+
+    ``active`` is the [tx, rx] boolean channel mask (a channel with no valid
+    frequency is zeroed before the sum).  Inputs may be host or device arrays;
+    the result is a device tensor.
+    """
+    if device is None:
+        device = pressure.device if torch.is_tensor(pressure) and pressure.is_cuda else torch.device("cuda", 0)
+    x = torch.as_tensor(pressure, device=device)
+    if x.dim() == 2:
+        x = x.unsqueeze(-1)
+    if x.dtype not in (torch.float32, torch.float64) or x.dim() != 3:
+        raise WustError("WUST:InvalidInput", "RF must be real [time,rx,tx] pressure")
+    n_time, n_rx, n_tx = x.shape
+    t = torch.as_tensor(np.asarray(time_s, np.float64).ravel(), device=device)
+    f = np.ascontiguousarray(np.asarray(frequencies_hz, np.float64).ravel())
+    if t.numel() != n_time:
+        raise WustError("WUST:InvalidInput", "RF must be real [time,rx,tx] pressure")
+    xd = x.permute(2, 1, 0).contiguous()                 # storage [tx][rx][time]: MATLAB [time, rx, tx]
+    act = torch.as_tensor(np.asarray(_host(active), bool), device=device)
+    if tuple(act.shape) != (n_tx, n_rx):
+        raise WustError("WUST:InvalidInput", "RF mask must be [tx,rx] or [tx,rx,frequency]")
+    act = act.to(torch.uint8).t().contiguous()           # storage [rx][tx]: MATLAB [tx, rx]
+    out = torch.empty((f.size, n_rx, n_tx), dtype=torch.complex64, device=device)
+    ms = C.c_float()
+    dt = float(t[1] - t[0])
+    with torch.cuda.device(device):
+        N.check(N.load().wust_rf_dtft(
+            C.c_void_p(xd.data_ptr()), 1 if x.dtype == torch.float64 else 0, n_time, n_rx, n_tx,
+            C.c_void_p(t.data_ptr()), dt, f.ctypes.data_as(C.c_void_p), f.size, C.c_void_p(act.data_ptr()),
+            C.c_void_p(out.data_ptr()), C.byref(ms), C.c_void_p(torch.cuda.current_stream(device).cuda_stream)))
+    P = out.permute(2, 1, 0)                              # [tx, rx, frequency] view
+    return (P, ms.value) if return_kernel_ms else P
+
+
+def _pair_distance(a, b):
+    return np.hypot(a[:, 0][:, None] - b[:, 0][None, :], a[:, 1][:, None] - b[:, 1][None, :])
+
+
+def wust_prepare(rf, cfg):
+    """RF [time, rx, tx] -> canonical observations with an explicit grid.
+
+    ``rf``: dict with ``time_s``, ``pressure`` ([time, rx, tx], real), ``tx_xy_m``,
+    ``rx_xy_m`` (optionally ``simulation``); ``cfg``: the reference's fields
+    (grid, frequencies_hz, c_geom_mps, phase_correction, mask, data_units,
+    measurement_provenance).  No window and no normalization besides dt.
+    """
+    if not isinstance(cfg, dict) or set(cfg) != _RF_CFG:
+        raise WustError("WUST:InvalidInput", "Missing or unsupported RF preparation fields")
+    time_s = np.asarray(_host(rf["time_s"]), np.float64).ravel()
+    if time_s.size < 2 or not np.all(np.isfinite(time_s)):
+        raise WustError("WUST:InvalidInput", "Invalid sample times")
+    dt = time_s[1] - time_s[0]
+    if not (dt > 0 and np.all(np.abs(np.diff(time_s) - dt) <= dt * 1e-6)):
+        raise WustError("WUST:InvalidInput", "Nonuniform time")
+    f = np.asarray(cfg["frequencies_hz"], np.float64).ravel()
+    if not np.all(np.isfinite(f) & (f > 0) & (f < 0.5 / dt)):
+        raise WustError("WUST:InvalidInput", "Frequency outside saved Nyquist")
+    nt, nr, nf = np.shape(rf["tx_xy_m"])[0], np.shape(rf["rx_xy_m"])[0], f.size
+    pressure = rf["pressure"]
+    shape = tuple(pressure.shape)
+    shape3 = shape + (1,) * (3 - len(shape))
+    if (torch.is_tensor(pressure) and pressure.is_complex()) or np.iscomplexobj(pressure) \
+            or shape3 != (time_s.size, nr, nt):
+        raise WustError("WUST:InvalidInput", "RF must be real [time,rx,tx] pressure")
+    mask = np.asarray(_host(cfg["mask"]))
+    if not np.all((mask == 0) | (mask == 1)):
+        raise WustError("WUST:InvalidInput", "Mask must contain 0/1")
+    mask = mask.astype(bool)
+    if mask.shape == (nt, nr):
+        mask = np.repeat(mask[:, :, None], nf, axis=2)
+    if _three(mask).shape != (nt, nr, nf):
+        raise WustError("WUST:InvalidInput", "RF mask must be [tx,rx] or [tx,rx,frequency]")
+    mask = _three(mask)
+    P = rf_dtft(pressure if torch.is_tensor(pressure) else np.asarray(pressure).reshape(shape3), time_s, f,
+                mask.any(axis=2))
+    inp = dict(schema="wust.frequency_input", schema_version=2, grid=cfg["grid"], pressure=P.cpu().numpy(),
+               mask=mask, pressure_axes="tx,rx,frequency", mask_axes="tx,rx,frequency", frequencies_hz=f,
+               tx_xy_m=rf["tx_xy_m"], rx_xy_m=rf["rx_xy_m"], fourier_sign=-1, pressure_type="total_pressure",
+               data_units=cfg["data_units"], spectrum_normalization="dtft_dt",
+               measurement_provenance=cfg["measurement_provenance"])
+    obs = wust_prepare_observations(inp)
+    if cfg["phase_correction"] not in ("none", "homogeneous-tof"):
+        raise WustError("WUST:InvalidInput", "Unsupported phase policy")
+    if cfg["phase_correction"] == "homogeneous-tof":
+        c = cfg["c_geom_mps"]
+        if not (np.ndim(c) == 0 and np.isfinite(c) and c > 0):
+            raise WustError("WUST:InvalidInput", "c_geom_mps must be a positive finite scalar")
+        measured = _pair_distance(_host(obs["tx_xy_m"]).astype(np.float64), _host(obs["rx_xy_m"]).astype(np.float64))
+        snapped = _pair_distance(obs["tx_snapped_xy_m"], obs["rx_snapped_xy_m"])
+        delay = (snapped - measured) / c
+        phase = np.exp(-2j * np.pi * delay[:, :, None] * obs["frequencies_hz"][None, None, :])
+        obs["Y"] = obs["Y"] * phase.astype(obs["Y"].dtype)     # single data times the rounded phase, as MATLAB
+        obs["frequency_data_norm"] = np.sum(np.abs(obs["Y"].astype(np.complex128)) ** 2, axis=(0, 1))
+    prep = dict(obs["preparation"])
+    prep.update(sampling_interval_s=dt, mask_shape=mask.shape, valid_mask_elements=int(np.count_nonzero(mask)),
+                phase_correction=cfg["phase_correction"])
+    if cfg["phase_correction"] == "homogeneous-tof":
+        prep["geometry_reference_speed_mps"] = cfg["c_geom_mps"]
+    obs["preparation"] = prep
+    obs["saved_dt_s"] = dt
+    if "simulation" in rf:
+        obs["simulation"] = rf["simulation"]
+    return obs

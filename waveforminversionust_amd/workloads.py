@@ -10,6 +10,9 @@ reference's own tests:
 * ``runtime_test_case()``      -- the 31x31 fixture of
   Runtime/tests/test_runtime.m:10-28 / Runtime/tests/wust_test_measurements.m:14.
 * ``probe_case()``             -- the 7x7 probe of Runtime/matlab/wust_probe.m:17-22.
+* ``continuation_cases()``     -- eight frequency stages 0.3-1.0 MHz, each on its own grid at
+  6 points per wavelength (201^2 ... 669^2 interiors + 24-cell PML), 256 TX each.
+* ``breast_case()``            -- 1088^2 nodes, 512-element ring, 12 frequencies, breast phantom.
 
 Geometry follows Runtime/matlab/wust_geometry.m: pixel-edge origin, node
 coordinates at pixel centres, nearest-node snapping with the lowest index on
@@ -152,6 +155,49 @@ def runtime_test_case():
                stencil_bounds=[1400.0, 1700.0], source_batch_size=2)
     return Case(name="runtime31", ny=31, nx=31, dx=1e-3, x_m=gx, y_m=gx.copy(), tx_index=tx,
                 rx_index=rx, c_true=c, frequencies_hz=[1e5, 1.2e5], cfg=cfg, tx_xy=tx_xy, rx_xy=rx_xy)
+
+
+def continuation_cases(n_stages=8, f_lo=0.3e6, f_hi=1.0e6, domain_m=0.156, pml_cells=24, n_elem=256,
+                       radius=70e-3, points_per_wavelength=6, c_min=1400.0):
+    """Frequency-continuation stages, each on its own grid: dx = c_min / (ppw f), a domain_m-wide
+    interior plus a pml_cells PML, the same 256-element ring and two-disc model sampled per grid.
+    With the defaults the interiors run from 201^2 (0.3 MHz, dx 0.78 mm) to 669^2 (1.0 MHz, dx 0.23 mm)."""
+    out = []
+    for f in np.linspace(f_lo, f_hi, n_stages):
+        dx = c_min / (points_per_wavelength * f)
+        interior = int(round(domain_m / dx))
+        out.append(ring_case(f"stage_{f / 1e6:.1f}MHz_{interior + 2 * pml_cells}", interior, pml_cells, dx, n_elem,
+                             radius, [float(f)], "discs"))
+    return out
+
+
+def breast_phantom(x, y, seed=7, n_inclusions=6):
+    """1480 m/s fat, a 1540 m/s fibroglandular ellipse (40 x 30 mm), and seeded round inclusions
+    of 1450-1580 m/s with radii 2-6 mm placed inside the ellipse's neighbourhood (r < 35 mm)."""
+    X, Y = np.meshgrid(x, y)
+    c = np.full(X.shape, 1480.0)
+    c[(X / 20e-3) ** 2 + (Y / 15e-3) ** 2 <= 1] = 1540.0
+    rng = np.random.default_rng(seed)
+    for _ in range(n_inclusions):
+        r = rng.uniform(2e-3, 6e-3)
+        rho, th = 35e-3 * np.sqrt(rng.uniform()), rng.uniform(0, 2 * np.pi)
+        c[(X - rho * np.cos(th)) ** 2 + (Y - rho * np.sin(th)) ** 2 <= r * r] = rng.uniform(1450.0, 1580.0)
+    return c
+
+
+def breast_case(seed=7):
+    """1024^2 interior + 32-cell PML (1088^2 nodes), dx 0.125 mm, 512-element ring at 60 mm,
+    12 frequencies 0.3-1.2 MHz, breast phantom."""
+    n, pml, dx = 1024, 32, 0.125e-3
+    x, _ = grid_axes(n + 2 * pml, dx)
+    pos = ring(512, 60e-3)
+    tx = snap(pos, x, x)
+    cfg = dict(backend="gpu", wavenumber="continuum", pml_strength=10.0, pml_m=pml * dx,
+               stencil_bounds=[1400.0, 1700.0], source_batch_size=512)
+    return Case(name="breast1088_512tx_12f", ny=n + 2 * pml, nx=n + 2 * pml, dx=dx, x_m=x, y_m=x.copy(),
+                tx_index=tx, rx_index=tx.copy(), c_true=breast_phantom(x, x, seed),
+                frequencies_hz=[float(f) for f in np.linspace(0.3e6, 1.2e6, 12)], cfg=cfg, tx_xy=pos,
+                rx_xy=pos.copy())
 
 
 def probe_case():
