@@ -82,6 +82,7 @@ def synthetic_obs(G, case, dev, tol):
     rx0 = torch.as_tensor(case.rx_index - 1, device=dev)
     opts = dict(backend="gpu", stencil_bounds=case.cfg["stencil_bounds"], tol=tol, device=dev)
     for fi, f in enumerate(case.frequencies_hz):
+        log(f"observations: forward solve {fi + 1}/{len(case.frequencies_hz)} at {f / 1e6:.3f} MHz")
         truth = G.HelmholtzSolver(case.x_m, case.y_m, torch.as_tensor(case.c_true, device=dev), None, f, -1,
                                   case.cfg["pml_strength"], case.cfg["pml_m"], opts)
         src = torch.zeros((ntx, nx, ny), dtype=torch.complex128, device=dev)
@@ -243,7 +244,7 @@ def run_breast(args):
     init = torch.full((case.ny, case.nx), 1500.0, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = Rc.wust_reconstruct(obs, init, cfg)
+    res = Rc.wust_reconstruct(obs, init, cfg, progress=log)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     rec = res["records"]

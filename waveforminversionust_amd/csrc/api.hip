@@ -187,7 +187,7 @@ struct wust_helmholtz {
         } else {
             s32 = dalloc<cf>(cnt);
             d32 = dalloc<cf>(cnt);
-            b32 = dalloc<cf>(2 * (int64_t)op.npad * Rpad);
+            b32 = dalloc<cf>(solve_work_elems(op, Rpad));
         }
         norms = dalloc<double>(2 * (int64_t)Rpad);
         loss_r = dalloc<double>(Rpad);

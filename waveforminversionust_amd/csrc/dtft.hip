@@ -21,7 +21,7 @@
 // X is read once from HBM (the kernel is HBM-bound for double input and
 // balanced with FP64 MFMA for single input at ~12 frequencies); K is a small
 // L2-resident table staged through LDS one sample chunk at a time and shared
-// by the four waves of a workgroup.
+// by the eight waves of a workgroup.
 #include "dtft.h"
 
 namespace wust {
@@ -30,8 +30,8 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-constexpr int DT_WAVES = 4;        // waves per workgroup, 16 columns each
-constexpr int DT_KC = 64;          // samples per LDS chunk (4 k-blocks of 16)
+constexpr int DT_WAVES = 8;        // waves per workgroup, 16 channels each
+constexpr int DT_KC = 64;          // samples per chunk (4 k-blocks of 16)
 constexpr int DT_LDS_PAD = 2;      // doubles of padding per LDS row
 
 // K[row][t] for row tiles of 8 frequencies (re rows 0-7, im rows 8-15), zero past
@@ -56,15 +56,52 @@ __global__ void dtft_table_kernel(const double *time_s, const double *freqs, int
     K[i] = v;
 }
 
-template <typename T>
-__device__ __forceinline__ double to_d(T v) {
-    return (double)v;
+// This lane's 16 samples of one chunk, t0 + 16 b + 4 kq + s (b = k-block, s = MFMA step), as
+// stored (float or double); zero for an inactive channel or past n_time.  VEC: n_time is a
+// multiple of 4, so each group of four is contiguous, aligned, and either all in range or not.
+template <typename T, bool VEC>
+__device__ __forceinline__ void load_chunk(const T *xc, int64_t t0, int kq, int64_t n_time, bool active, T (&v)[4][4]) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int64_t t = t0 + 16 * b + 4 * kq;
+        if (VEC) {
+            const bool in = active && t < n_time;
+            const T *p = xc + (in ? t : 0);
+            if (sizeof(T) == 4) {
+                const float4 u = *reinterpret_cast<const float4 *>(p);
+                v[b][0] = in ? (T)u.x : (T)0;
+                v[b][1] = in ? (T)u.y : (T)0;
+                v[b][2] = in ? (T)u.z : (T)0;
+                v[b][3] = in ? (T)u.w : (T)0;
+            } else {
+                const double2 u0 = reinterpret_cast<const double2 *>(p)[0];
+                const double2 u1 = reinterpret_cast<const double2 *>(p)[1];
+                v[b][0] = in ? (T)u0.x : (T)0;
+                v[b][1] = in ? (T)u0.y : (T)0;
+                v[b][2] = in ? (T)u1.x : (T)0;
+                v[b][3] = in ? (T)u1.y : (T)0;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const bool in = active && t + s < n_time;
+                const T u = xc[in ? t + s : 0];
+                v[b][s] = in ? u : (T)0;
+            }
+        }
+    }
 }
 
-// One wave: 16 columns x RT row tiles over all samples.  Workgroup = DT_WAVES waves.
-template <int RT, typename T>
+// One wave: 16 channels x RT row tiles over all samples; a workgroup of DT_WAVES waves shares
+// the kernel table, double-buffered in LDS (one barrier per chunk; the next chunk's table and
+// samples are in flight while the current chunk's MFMAs run).
+template <int RT, typename T, bool VEC>
 __global__ void __launch_bounds__(64 * DT_WAVES) dtft_kernel(DtftParams p) {
-    __shared__ double ksh[RT * 16][DT_KC + DT_LDS_PAD];
+    constexpr int ROWS = RT * 16;
+    constexpr int NT = 64 * DT_WAVES;
+    constexpr int PER_THREAD = ROWS * DT_KC / NT;   // table doubles staged by each thread
+    static_assert(PER_THREAD * NT == ROWS * DT_KC, "table chunk must split evenly over the threads");
+    __shared__ double ksh[2][ROWS][DT_KC + DT_LDS_PAD];
     const T *x = static_cast<const T *>(p.rf);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, kq = lane >> 4;
@@ -74,37 +111,57 @@ __global__ void __launch_bounds__(64 * DT_WAVES) dtft_kernel(DtftParams p) {
     const int tx = (int)(colc / p.n_rx), rx = (int)(colc - (int64_t)tx * p.n_rx);
     const bool active = col_ok && p.active[tx + (int64_t)p.n_tx * rx] != 0;
     const T *xc = x + colc * p.n_time;
+    // table staging: element q * NT + threadIdx.x of the [ROWS][DT_KC] chunk (coalesced rows)
+
     v4d acc[RT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) acc[r] = v4d{0.0, 0.0, 0.0, 0.0};
     bool bad = false;
-    for (int64_t t0 = 0; t0 < p.nt_pad; t0 += DT_KC) {
-        // this lane's samples of the chunk: t0 + 16 b + 4 kq + s, b = k-block, s = MFMA step
-        double xv[DT_KC / 16][4];
+    T xv[4][4], xn[4][4];
+    double kr[PER_THREAD];
+    // prologue: chunk 0
+    load_chunk<T, VEC>(xc, 0, kq, p.n_time, active, xv);
 #pragma unroll
-        for (int b = 0; b < DT_KC / 16; ++b)
+    for (int q = 0; q < PER_THREAD; ++q) {
+        const int e = q * NT + threadIdx.x, row = e / DT_KC, tt = e - row * DT_KC;
+        ksh[0][row][tt] = p.K[(int64_t)row * p.nt_pad + tt];
+    }
+    __syncthreads();
+    const int64_t nchunks = p.nt_pad / DT_KC;
+    for (int64_t ch = 0; ch < nchunks; ++ch) {
+        const int cur = (int)(ch & 1);
+        const bool more = ch + 1 < nchunks;
+        if (more) {
+            load_chunk<T, VEC>(xc, (ch + 1) * DT_KC, kq, p.n_time, active, xn);
+#pragma unroll
+            for (int q = 0; q < PER_THREAD; ++q) {
+                const int e = q * NT + threadIdx.x, row = e / DT_KC, tt = e - row * DT_KC;
+                kr[q] = p.K[(int64_t)row * p.nt_pad + (ch + 1) * DT_KC + tt];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const int64_t t = t0 + 16 * b + 4 * kq + s;
-                const bool in = active && t < p.n_time;
-                const double v = to_d(xc[in ? t : 0]);
-                if (in && !isfinite(v)) bad = true;
-                xv[b][s] = in ? v : 0.0;
-            }
-        __syncthreads();   // previous chunk's table reads are done
-        for (int e = threadIdx.x; e < RT * 16 * DT_KC; e += 64 * DT_WAVES) {
-            const int row = e / DT_KC, tt = e - row * DT_KC;
-            ksh[row][tt] = p.K[(int64_t)row * p.nt_pad + t0 + tt];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int b = 0; b < DT_KC / 16; ++b)
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
+                const double xd = (double)xv[b][s];
+                bad |= !isfinite(xd);
 #pragma unroll
                 for (int r = 0; r < RT; ++r)
-                    acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ksh[16 * r + c][16 * b + 4 * kq + s], xv[b][s],
+                    acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ksh[cur][16 * r + c][16 * b + 4 * kq + s], xd,
                                                                   acc[r], 0, 0, 0);
+            }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < PER_THREAD; ++q) {
+                const int e = q * NT + threadIdx.x, row = e / DT_KC, tt = e - row * DT_KC;
+                ksh[cur ^ 1][row][tt] = kr[q];
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) xv[b][s] = xn[b][s];
+        }
+        __syncthreads();
     }
     if (__any(bad) && lane == 0) atomicOr(p.status, 1u);
     // registers (q, q + 2) of lane c + 16 kq: real and imaginary parts of frequency 8 r + kq + 4 q
@@ -120,13 +177,13 @@ __global__ void __launch_bounds__(64 * DT_WAVES) dtft_kernel(DtftParams p) {
         }
 }
 
-template <typename T>
+template <typename T, bool VEC>
 void launch_rt(int rt, const DtftParams &p, dim3 grid, hipStream_t s) {
     switch (rt) {
-        case 1: hipLaunchKernelGGL((dtft_kernel<1, T>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
-        case 2: hipLaunchKernelGGL((dtft_kernel<2, T>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
-        case 3: hipLaunchKernelGGL((dtft_kernel<3, T>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
-        default: hipLaunchKernelGGL((dtft_kernel<4, T>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
+        case 1: hipLaunchKernelGGL((dtft_kernel<1, T, VEC>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
+        case 2: hipLaunchKernelGGL((dtft_kernel<2, T, VEC>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
+        case 3: hipLaunchKernelGGL((dtft_kernel<3, T, VEC>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
+        default: hipLaunchKernelGGL((dtft_kernel<4, T, VEC>), grid, dim3(64 * DT_WAVES), 0, s, p); break;
     }
     WUST_LAUNCH_CHECK();
 }
@@ -182,8 +239,14 @@ void rf_dtft(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx,
         p.f0 = f0;
         p.n_freq_chunk = nf;
         if (kernel_ms) WUST_HIP_CHECK(hipEventRecord(e0, s));
-        if (is_double) launch_rt<double>(rt, p, grid, s);
-        else launch_rt<float>(rt, p, grid, s);
+        const bool vec = n_time % 4 == 0;
+        if (is_double) {
+            if (vec) launch_rt<double, true>(rt, p, grid, s);
+            else launch_rt<double, false>(rt, p, grid, s);
+        } else {
+            if (vec) launch_rt<float, true>(rt, p, grid, s);
+            else launch_rt<float, false>(rt, p, grid, s);
+        }
         if (kernel_ms) {
             WUST_HIP_CHECK(hipEventRecord(e1, s));
             WUST_HIP_CHECK(hipEventSynchronize(e1));

@@ -51,6 +51,8 @@ void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s);
 // ---- complex-FP32 block solve: out = H^{-1} src (or H^{-H} src), solve layout ----
 // src, out: [m][npad][Rpad] complex-FP32; out must not alias src; scratch: 2*npad*Rpad.
 // Rpad must be a multiple of 16.
-void solve_engine(const Operator &op, const cf *src, cf *out, cf *scratch, int Rpad, bool adjoint, hipStream_t s);
+// work: [m][npad][Rpad] forward iterates + 4 operand blocks [npad][Rpad] (solve_work_elems)
+void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad, bool adjoint, hipStream_t s);
+inline int64_t solve_work_elems(const Operator &op, int Rpad) { return ((int64_t)op.m + 4) * op.npad * Rpad; }
 
 }  // namespace wust

@@ -15,6 +15,8 @@
 //            (H^H has U_{j-1}^H below and L_{j+1}^H above the diagonal).
 // Layout of every field block: [npad rows][Rpad RHS], RHS fastest, so the
 // coupling loads and the output stores are coalesced across RHS.
+#include <cstdlib>
+
 #include "cgemm.h"
 #include "engine.h"
 #include "profile.h"
@@ -142,9 +144,289 @@ __global__ void __launch_bounds__(256) sweep_gemm_kernel(SweepParams p) {
 
 constexpr int SWEEP_RB = 1;   // 16-row tiles: 2 * 36 * 16 = 1152 WGs at 560^2 x 256 RHS
 
+// ---- fused sweep step: GEMM + the next step's operand in the epilogue ---------------------
+//
+// The next step's operand B' = src' - P v needs rows k-1, k, k+1 of this step's result v.
+// Each workgroup therefore computes a 16-row MFMA tile that overlaps its neighbours by one
+// row on each side and owns its 14 middle rows: strip mb owns rows 14 mb .. 14 mb + 13 and
+// computes rows i0 .. i0 + 15 with i0 = clamp(14 mb - 1, 0, npad - 16), so every owned row
+// has both neighbours in the tile (or outside the block, where they are zero).  The owned
+// rows of v are stored and B' is formed from the tile in LDS; the separate operand pass and
+// its launch disappear, at the cost of 16/14 of the MFMA work.  Back substitution reads
+// its base (y_j) from a different array than it writes (u_j), so halo rows are never read
+// after a neighbour rewrote them.
+constexpr int FUSED_OWN = 14;
+
+struct FusedTask {
+    const cf *X;              // inverse page
+    const cf *B;              // operand block [npad][Rpad]
+    const cf *base;           // null, or the block added to op(X) B
+    cf *out;                  // result block
+    cf *Bn;                   // null, or the next step's operand block
+    const cf *nsrc;           // next step's source block (or null)
+    const cf *cn, *cc, *cs;   // coupling acting on this step's result
+    int transposed;
+};
+struct FusedParams {
+    FusedTask task[2];
+    int ntasks, npad, Rpad;
+    int strips, units, per_xcd;
+};
+
+template <int ALAY>
+__global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int L = xcd * p.per_xcd + local;
+    if (L >= p.units) return;
+    const int npad = p.npad, Rpad = p.Rpad;
+    const int Nb = Rpad / 16;
+    const int nb = L % Nb;
+    const int mb = (L / Nb) % p.strips;
+    const int task = L / (Nb * p.strips);
+    const FusedTask &tk = p.task[task];
+    const int o0 = FUSED_OWN * mb;
+    const int o1 = min(o0 + FUSED_OWN - 1, npad - 1);
+    const int i0 = min(max(o0 - 1, 0), npad - 16);
+    const int j0 = nb * 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+
+    CgemmWave<1, 1, ALAY, B_ROW, false> acc;
+    acc.zero();
+    const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
+    acc.run(A, npad, tk.B + j0, Rpad, wave, npad / 16, 4, g, c);
+
+    __shared__ float4 red[3][2][64];
+    __shared__ cf tile[16][17];
+    if (wave > 0) {
+        float re[4], im[4];
+        acc.result(0, 0, re, im);
+        red[wave - 1][0][lane] = make_float4(re[0], re[1], re[2], re[3]);
+        red[wave - 1][1][lane] = make_float4(im[0], im[1], im[2], im[3]);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        float re[4], im[4];
+        acc.result(0, 0, re, im);
+#pragma unroll
+        for (int w = 0; w < 3; ++w) {
+            const float4 u = red[w][0][lane], v = red[w][1][lane];
+            re[0] += u.x; re[1] += u.y; re[2] += u.z; re[3] += u.w;
+            im[0] += v.x; im[1] += v.y; im[2] += v.z; im[3] += v.w;
+        }
+        // lane holds tile rows 4 g + r of column j0 + c
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cf v = mk(re[r], im[r]);
+            if (tk.base) v = v + tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
+            tile[4 * g + r][c] = v;
+        }
+    }
+    __syncthreads();
+    // one element per thread: tile row t, column j0 + cc
+    const int t = threadIdx.x >> 4, cc = threadIdx.x & 15;
+    const int k = i0 + t;
+    if (k < o0 || k > o1) return;
+    const int64_t off = (int64_t)k * Rpad + j0 + cc;
+    const cf v = tile[t][cc];
+    tk.out[off] = v;
+    if (!tk.Bn) return;
+    cf b = tk.nsrc ? tk.nsrc[off] : mk(0.f, 0.f);
+    const cf vn = (k >= 1) ? tile[t - 1][cc] : mk(0.f, 0.f);        // row k - 1 (in the tile when k >= 1)
+    const cf vs = (k + 1 < npad) ? tile[t + 1][cc] : mk(0.f, 0.f);  // row k + 1 (in the tile when < npad)
+    cf a;
+    if (!tk.transposed) {
+        a = tk.cc[k] * v;
+        if (k >= 1) cfma(a, tk.cn[k], vn);
+        if (k + 1 < npad) cfma(a, tk.cs[k], vs);
+    } else {
+        a = conjf_(tk.cc[k]) * v;
+        if (k + 1 < npad) cfma(a, conjf_(tk.cn[k + 1]), vs);
+        if (k >= 1) cfma(a, conjf_(tk.cs[k - 1]), vn);
+    }
+    tk.Bn[off] = b - a;
+}
+
 }  // namespace
 
-void solve_engine(const Operator &op, const cf *src, cf *out, cf *scratch, int Rpad, bool adjoint, hipStream_t s) {
+static void solve_engine_unfused(const Operator &op, const cf *src, cf *out, cf *scratch, int Rpad, bool adjoint,
+                                 hipStream_t s);
+
+void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad, bool adjoint, hipStream_t s) {
+    static const bool unfused = [] {
+        const char *e = std::getenv("WUST_SWEEP_UNFUSED");
+        return e && e[0] == '1';
+    }();
+    const int m = op.m, p = op.p, npad = op.npad;
+    const int64_t blk = (int64_t)npad * Rpad;
+    cf *yf = work;                                 // forward iterates y_j / z_j: [m][npad][Rpad]
+    cf *ops = work + (int64_t)m * blk;             // operand blocks: [2 chains][2 ping-pong]
+    if (unfused) {
+        solve_engine_unfused(op, src, out, ops, Rpad, adjoint, s);
+        return;
+    }
+    auto S = [&](int j) { return src + j * blk; };
+    auto Y = [&](int j) { return yf + j * blk; };
+    auto O = [&](int j) { return out + j * blk; };
+    auto OP = [&](int chain, int pp) { return ops + (int64_t)(2 * chain + pp) * blk; };
+    int Ld[3] = {DNW, DW, DSW};
+    int Ud[3] = {DNE, DE, DSE};
+    const int strips = (npad + FUSED_OWN - 1) / FUSED_OWN;
+    auto couple = [&](FusedTask &tk, const int dirs[3], int cblock, bool transposed) {
+        tk.cn = op.coef(dirs[0], cblock);
+        tk.cc = op.coef(dirs[1], cblock);
+        tk.cs = op.coef(dirs[2], cblock);
+        tk.transposed = transposed ? 1 : 0;
+    };
+    auto launch = [&](std::vector<FusedTask> &tasks) {
+        if (tasks.empty()) return;
+        FusedParams fp{};
+        fp.ntasks = (int)tasks.size();
+        for (int i = 0; i < fp.ntasks; ++i) fp.task[i] = tasks[i];
+        fp.npad = npad;
+        fp.Rpad = Rpad;
+        fp.strips = strips;
+        fp.units = fp.ntasks * strips * (Rpad / 16);
+        fp.per_xcd = (fp.units + 7) / 8;
+        if (adjoint)
+            launch_k(PC_SWEEP_GEMM, sweep_fused_kernel<A_CONJT>, dim3(8 * fp.per_xcd), dim3(256), 0, s, fp);
+        else
+            launch_k(PC_SWEEP_GEMM, sweep_fused_kernel<A_COL>, dim3(8 * fp.per_xcd), dim3(256), 0, s, fp);
+    };
+    const int nl = p, nr = m - 1 - p;
+    const int nsteps = nl > nr ? nl : nr;
+    // elimination from both ends: y_j = X_j B_j, and B_{next} from y_j in the epilogue
+    for (int st = 0; st < nsteps; ++st) {
+        std::vector<FusedTask> tasks;
+        if (st < nl) {
+            const int j = st;
+            FusedTask tk{};
+            tk.X = op.page(j);
+            tk.B = (st == 0) ? S(j) : OP(0, st & 1);
+            tk.out = Y(j);
+            if (st + 1 < nl) {                       // next left block j + 1 (not the meeting block)
+                tk.Bn = OP(0, (st + 1) & 1);
+                tk.nsrc = S(j + 1);
+                if (!adjoint) couple(tk, Ld, j + 1, false);    // - L_{j+1} y_j
+                else couple(tk, Ud, j, true);                  // - U_j^H y_j
+            }
+            tasks.push_back(tk);
+        }
+        if (st < nr) {
+            const int j = m - 1 - st;
+            FusedTask tk{};
+            tk.X = op.page(j);
+            tk.B = (st == 0) ? S(j) : OP(1, st & 1);
+            tk.out = Y(j);
+            if (st + 1 < nr) {                       // next right block j - 1
+                tk.Bn = OP(1, (st + 1) & 1);
+                tk.nsrc = S(j - 1);
+                if (!adjoint) couple(tk, Ud, j - 1, false);    // - U_{j-1} z_j
+                else couple(tk, Ld, j, true);                  // - L_j^H z_j
+            }
+            tasks.push_back(tk);
+        }
+        launch(tasks);
+    }
+    // meeting block u_p = X_p (b_p - L_p y_{p-1} - U_p z_{p+1}), and the operands of the first
+    // back-substitution steps, u_p's couplings: operand passes + the plain GEMM
+    {
+        auto term = [&](SweepTask &tk, const int dirs[3], int cblock, const cf *v, bool transposed) {
+            SweepTerm &t = tk.t[tk.nterms++];
+            t.cn = op.coef(dirs[0], cblock);
+            t.cc = op.coef(dirs[1], cblock);
+            t.cs = op.coef(dirs[2], cblock);
+            t.v = v;
+            t.transposed = transposed ? 1 : 0;
+        };
+        auto run = [&](std::vector<SweepTask> &tasks, bool gemm) {
+            SweepParams sp{};
+            sp.ntasks = (int)tasks.size();
+            for (int i = 0; i < sp.ntasks; ++i) sp.task[i] = tasks[i];
+            sp.npad = npad;
+            sp.Rpad = Rpad;
+            const int64_t nops = (int64_t)sp.ntasks * blk;
+            launch_k(PC_SWEEP_OPERAND, sweep_operand_kernel, dim3((unsigned)((nops + 255) / 256)), dim3(256), 0, s,
+                     sp);
+            if (!gemm) return;
+            sp.units = sp.ntasks * (npad / (16 * SWEEP_RB)) * (Rpad / 16);
+            sp.per_xcd = (sp.units + 7) / 8;
+            if (adjoint)
+                launch_k(PC_SWEEP_GEMM, sweep_gemm_kernel<SWEEP_RB, A_CONJT>, dim3(8 * sp.per_xcd), dim3(256), 0, s,
+                         sp);
+            else
+                launch_k(PC_SWEEP_GEMM, sweep_gemm_kernel<SWEEP_RB, A_COL>, dim3(8 * sp.per_xcd), dim3(256), 0, s, sp);
+        };
+        std::vector<SweepTask> mid(1);
+        SweepTask &tk = mid[0];
+        tk = SweepTask{};
+        tk.X = op.page(p);
+        tk.xconj = adjoint ? 1 : 0;
+        tk.src = S(p);
+        tk.B = OP(0, 0);
+        tk.out = O(p);
+        if (p >= 1) {
+            if (!adjoint) term(tk, Ld, p, Y(p - 1), false);
+            else term(tk, Ud, p - 1, Y(p - 1), true);
+        }
+        if (p + 1 <= m - 1) {
+            if (!adjoint) term(tk, Ud, p, Y(p + 1), false);
+            else term(tk, Ld, p + 1, Y(p + 1), true);
+        }
+        run(mid, true);
+        std::vector<SweepTask> first;
+        if (nl >= 1) {                                 // left j = p - 1: - X_j U_j u_p
+            SweepTask t{};
+            t.B = OP(0, 1);
+            if (!adjoint) term(t, Ud, p - 1, O(p), false);
+            else term(t, Ld, p, O(p), true);
+            first.push_back(t);
+        }
+        if (nr >= 1) {                                 // right j = p + 1: - X_j L_j u_p
+            SweepTask t{};
+            t.B = OP(1, 1);
+            if (!adjoint) term(t, Ld, p + 1, O(p), false);
+            else term(t, Ud, p, O(p), true);
+            first.push_back(t);
+        }
+        if (!first.empty()) run(first, false);
+    }
+    // back substitution outward: u_j = y_j + X_j B_j with B_j = - (coupling) u_{j +- 1}
+    for (int st = 0; st < nsteps; ++st) {
+        std::vector<FusedTask> tasks;
+        if (st < nl) {
+            const int j = p - 1 - st;
+            FusedTask tk{};
+            tk.X = op.page(j);
+            tk.B = OP(0, (st + 1) & 1);
+            tk.base = Y(j);
+            tk.out = O(j);
+            if (st + 1 < nl) {                       // next left block j - 1
+                tk.Bn = OP(0, st & 1);
+                if (!adjoint) couple(tk, Ud, j - 1, false);    // - U_{j-1} u_j
+                else couple(tk, Ld, j, true);                  // - L_j^H u_j
+            }
+            tasks.push_back(tk);
+        }
+        if (st < nr) {
+            const int j = p + 1 + st;
+            FusedTask tk{};
+            tk.X = op.page(j);
+            tk.B = OP(1, (st + 1) & 1);
+            tk.base = Y(j);
+            tk.out = O(j);
+            if (st + 1 < nr) {                       // next right block j + 1
+                tk.Bn = OP(1, st & 1);
+                if (!adjoint) couple(tk, Ld, j + 1, false);    // - L_{j+1} u_j
+                else couple(tk, Ud, j, true);                  // - U_j^H u_j
+            }
+            tasks.push_back(tk);
+        }
+        launch(tasks);
+    }
+}
+
+static void solve_engine_unfused(const Operator &op, const cf *src, cf *out, cf *scratch, int Rpad, bool adjoint,
+                                 hipStream_t s) {
     const int m = op.m, p = op.p, npad = op.npad;
     const int64_t blk = (int64_t)npad * Rpad;
     auto S = [&](int j) { return src + j * blk; };

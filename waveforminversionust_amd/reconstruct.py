@@ -282,12 +282,14 @@ def _frequency_data_norm(obs):
         return np.sum(np.abs(Y.astype(np.complex128)) ** 2, axis=(0, 1))
 
 
-def wust_reconstruct(obs, initial, cfg):
+def wust_reconstruct(obs, initial, cfg, progress=None):
     """Frequency-continuation FWI in slowness with clipped PR/FR NCG.
 
     Returns a dict with the reference's result fields (``c_mps`` as a torch
     [ny, nx] tensor on the GPU, ``records`` a list of dicts, completion and
     schedule bookkeeping, ``final_data_residual`` None when not evaluated).
+    ``progress`` (optional callable) receives the reference's per-step line
+    "FWI k/n f=... MHz residual(pre)=... wall=...s".
     """
     t_all = time.perf_counter()
     cfg = prepare_reconstruction(obs, initial, cfg)
@@ -340,6 +342,10 @@ def wust_reconstruct(obs, initial, cfg):
             forward_rhs=state.forward_rhs, adjoint_rhs=state.adjoint_rhs,
             linearized_rhs=int(np.asarray(obs["tx_index"]).size), relative_slowness_update=rel,
             stage_id=fi, loss_model_step=step))
+        if progress is not None:
+            r = records[-1]
+            progress(f"FWI {step + 1}/{len(schedule)} f={freqs[fi] / 1e6:.3f} MHz "
+                     f"residual(pre)={r['relative_residual_before_update']:.6g} wall={r['wall_seconds']:.3f}s")
         del state, solver
     n = len(schedule)
     reason, category = "schedule_complete", "completion"
