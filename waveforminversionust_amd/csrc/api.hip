@@ -145,6 +145,7 @@ struct wust_helmholtz {
         dfree(op.pbuf);
         dfree(op.nswork);
         dfree(op.ns_norm);
+        dfree(op.ns_rounds);
         dfree(op.status);
         dfree(idx);
     }
@@ -407,6 +408,7 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         if (mode != WUST_MODE_PLAIN && op.factor_method == FACTOR_AUTO) {
             op.nswork = dalloc<cf>(6 * pg);
             op.ns_norm = dalloc<float>(2 * 8);
+            op.ns_rounds = dalloc<int>(op.m);
         }
         op.factor_bytes = (int64_t)sizeof(cf) * pg * op.m;
         WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, 8 * sizeof(uint32_t), s));

@@ -31,6 +31,7 @@ struct Operator {
     cf *pbuf = nullptr;           // diagonal-block inverses handed between panel launches: [2 chains][2][32*32]
     cf *nswork = nullptr;         // Newton-Schulz iterates and cubic operand: [2 chains][3][npad*npad]
     float *ns_norm = nullptr;     // ||E_r||_F^2 per chain and round: [2 chains][8]
+    int *ns_rounds = nullptr;     // Newton-Schulz rounds used per block, + 64 if the last was forced: [m]
     uint32_t *status = nullptr;   // device status words: [0] flags, [2..3] pivot ratios (float),
                                   // [4] largest Newton-Schulz residual bound (float)
     double assembly_ms = 0, factor_ms = 0, min_pivot_ratio = 1.0;

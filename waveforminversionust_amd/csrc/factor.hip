@@ -22,6 +22,12 @@
 //     WUST_SINGULAR_SCHUR_BLOCK, and the smallest |pivot| / max|entry below
 //     it| inside each 16x16 pivot block is tracked so that a loss of
 //     dominance is visible.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -421,6 +427,7 @@ struct NsTask {
     cf *page;         // X_{r+1} when this round is the last
     cf *xnext;        // X_{r+1} otherwise
     float *norms;     // ||E_r||_F^2 per round (cleared by the Schur launch)
+    int *rounds;      // this block's slot of the rounds-used record
 };
 struct NsParams {
     NsTask task[2];
@@ -545,6 +552,8 @@ __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
         const float bound = cubic ? n * sqrtf(n) : n;
         if (!(bound <= p.accept)) atomicOr(p.status, (uint32_t)ST_INVERSE_UNCONVERGED);
         if (bound >= 0.f) atomicMax(reinterpret_cast<int *>(p.resid_max), __float_as_int(bound));
+        // rounds this block used; + 64 when the budget ran out before ||E||^2 reached c^2
+        *tk.rounds = (p.r + 1) + (n > p.c2 ? 64 : 0);
     }
 }
 
@@ -594,6 +603,27 @@ constexpr float NS_Q2 = 2e-6f;
 constexpr float NS_C2 = 1.5874e-4f;   // (2e-6)^(2/3)
 constexpr float NS_ACCEPT = 1e-4f;
 
+// Newton-Schulz round budgets learned per operator shape and frequency: the rounds each
+// chain step needed in the last factorization of that key (both chains' maximum; one more
+// where a block's budget ran out first).  A budget that proves too small costs accuracy
+// within the accepted bound or, beyond it, the Gauss-Jordan fallback of that one
+// factorization, and is raised for the next one.
+struct BudgetKey {
+    int ny, nx, npad, dispersion;
+    double f, dx, dy, a0, L;
+    bool operator<(const BudgetKey &o) const {
+        return std::tie(ny, nx, npad, dispersion, f, dx, dy, a0, L) <
+               std::tie(o.ny, o.nx, o.npad, o.dispersion, o.f, o.dx, o.dy, o.a0, o.L);
+    }
+};
+std::mutex g_budget_mu;
+std::map<BudgetKey, std::vector<uint8_t>> g_budgets;
+
+BudgetKey budget_key(const Operator &op) {
+    const AssemblyParams &a = op.ap;
+    return BudgetKey{a.ny, a.nx, a.npad, a.dispersion, a.f, a.dx, a.dy, a.a0, a.L};
+}
+
 struct FactorRun {
     Operator &op;
     hipStream_t s;
@@ -601,11 +631,13 @@ struct FactorRun {
     int pml_cells;
     int ns_blocks = 0;
     int update_cap = NS_KMAX;
+    std::vector<uint8_t> learned;   // per chain step; empty: the default budget
 
     // Newton-Schulz round budget of chain step `st`: the PML columns at the
     // start of each chain change fastest.
     int ns_budget(int st) const {
-        const int b = st < pml_cells + 8 ? NS_KMAX : 2;
+        int b = st < pml_cells + 8 ? NS_KMAX : 2;
+        if (st < (int)learned.size() && learned[st] >= 1) b = learned[st];
         return b < update_cap ? b : update_cap;
     }
 
@@ -693,6 +725,7 @@ struct FactorRun {
                     tk.xnext = xb + (r % 2) * pg;
                     tk.page = op.page(j);
                     tk.norms = op.ns_norm + c * NS_KMAX;
+                    tk.rounds = op.ns_rounds + j;
                 }
             };
             set_round(0);
@@ -743,10 +776,41 @@ void factor_engine(Operator &op, hipStream_t s) {
         const int c = std::atoi(cap);
         if (c >= 1 && c <= NS_KMAX) run.update_cap = c;
     }
+    static const bool learn = [] {
+        const char *e = std::getenv("WUST_NS_FIXED_BUDGET");
+        return !(e && e[0] == '1');
+    }();
+    const BudgetKey key = budget_key(op);
+    if (learn && run.allow_ns && op.ns_rounds) {
+        std::lock_guard<std::mutex> lk(g_budget_mu);
+        auto it = g_budgets.find(key);
+        if (it != g_budgets.end()) run.learned = it->second;
+    }
+    if (op.ns_rounds) WUST_HIP_CHECK(hipMemsetAsync(op.ns_rounds, 0, sizeof(int) * op.m, s));
     run.run();
     uint32_t st[6];
+    std::vector<int> used;
+    if (learn && run.allow_ns && op.ns_rounds) {
+        used.resize(op.m);
+        WUST_HIP_CHECK(hipMemcpyAsync(used.data(), op.ns_rounds, sizeof(int) * op.m, hipMemcpyDeviceToHost, s));
+    }
     WUST_HIP_CHECK(hipMemcpyAsync(st, op.status, sizeof(st), hipMemcpyDeviceToHost, s));
     WUST_HIP_CHECK(hipStreamSynchronize(s));
+    if (!used.empty()) {
+        // chain step st produced blocks st (left) and m - 1 - st (right)
+        const int m = op.m, nsteps = std::max(op.p, m - 1 - op.p);
+        std::vector<uint8_t> next((size_t)nsteps, 0);
+        for (int j = 0; j < m; ++j) {
+            if (j == op.p || used[j] == 0) continue;
+            const int stp = j < op.p ? j : m - 1 - j;
+            const bool forced = used[j] >= 64;
+            int need = (used[j] & 63) + (forced ? 1 : 0);
+            need = std::min(need, NS_KMAX);
+            next[stp] = (uint8_t)std::max<int>(next[stp], need);
+        }
+        std::lock_guard<std::mutex> lk(g_budget_mu);
+        g_budgets[key] = next;
+    }
     op.factor_fallback = 0;
     if (st[0] & ST_INVERSE_UNCONVERGED) {
         // some block's Newton-Schulz iteration missed its bound: redo every block by Gauss-Jordan
