@@ -213,10 +213,13 @@ def run_dtft(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64 (MFMA) from f32 RF, c64 result", "data": "synthetic",
         "config": {"workload": "dtft_256tx_256rx_4096t_12f", "n_tx": n_tx, "n_rx": n_rx, "n_time": n_time,
                    "n_freq": len(freqs)},
-        "roofline": {"kernel": "dtft_kernel", "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_launch_us": round(k_s * 1e6, 2), "algorithmic_bytes_per_launch": rf_bytes + out_bytes,
-                     "fp64_mfma_tflops": round(tfs, 2), "fp64_mfma_frac": round(tfs / 78.6, 4)},
+        # FP64 MFMA time (4 nf Nt N flops at 78.6 TFLOP/s, 0.16 ms) exceeds the HBM time of the RF stream
+        # (0.13 ms at 8 TB/s), so the kernel is priced against the FP64 MFMA peak
+        "roofline": {"kernel": "dtft_kernel", "bound": "mfma", "achieved": round(tfs, 2), "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": round(tfs / 78.6, 4), "traffic": None,
+                     "avg_launch_us": round(k_s * 1e6, 2), "algorithmic_flops_per_launch": flops,
+                     "algorithmic_bytes_per_launch": rf_bytes + out_bytes, "hbm_gbs": round(gbs, 1),
+                     "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <utility>
@@ -605,9 +606,12 @@ constexpr float NS_ACCEPT = 1e-4f;
 
 // Newton-Schulz round budgets learned per operator shape and frequency: the rounds each
 // chain step needed in the last factorization of that key (both chains' maximum; one more
-// where a block's budget ran out first).  A budget that proves too small costs accuracy
-// within the accepted bound or, beyond it, the Gauss-Jordan fallback of that one
-// factorization, and is raised for the next one.
+// where a block's budget ran out first).  The next factorization with that key uses exactly
+// that budget in the interior columns, where one cubic round suffices with a wide margin,
+// and one spare round in the PML columns, whose convergence depends more on the model.  A
+// budget that still proves too small costs accuracy within the accepted bound or, beyond
+// it, the Gauss-Jordan fallback of that factorization, after which the key starts again
+// from the default budgets.
 struct BudgetKey {
     int ny, nx, npad, dispersion;
     double f, dx, dy, a0, L;
@@ -635,9 +639,20 @@ struct FactorRun {
 
     // Newton-Schulz round budget of chain step `st`: the PML columns at the
     // start of each chain change fastest.
+    // Newton-Schulz or Gauss-Jordan for chain step `st`.  A Gauss-Jordan block costs about as
+    // much as 3-4 Newton-Schulz rounds (npad / 32 panel launches against three GEMM launches per
+    // round), so steps that needed 4 or more rounds, or whose need is unknown, use Gauss-Jordan;
+    // without a learned profile the second column of each chain (seeded from the first, the
+    // slowest to converge: 7-9 rounds at 6 points per wavelength) does too.
+    bool ns_step(int st) const {
+        if (learned.empty()) return st >= 2;
+        return st < (int)learned.size() && learned[st] >= 1 && learned[st] <= 3;
+    }
+
     int ns_budget(int st) const {
-        int b = st < pml_cells + 8 ? NS_KMAX : 2;
-        if (st < (int)learned.size() && learned[st] >= 1) b = learned[st];
+        const bool pml = st < pml_cells + 8;
+        int b = pml ? NS_KMAX : 2;
+        if (st < (int)learned.size() && learned[st] >= 1) b = std::min(learned[st] + (pml ? 1 : 0), NS_KMAX);
         return b < update_cap ? b : update_cap;
     }
 
@@ -649,7 +664,7 @@ struct FactorRun {
         float *ratio_d = reinterpret_cast<float *>(op.status + 2);
         std::vector<int> gj, ns;
         for (int c = 0; c < (int)steps.size(); ++c) {
-            const bool use_ns = allow_ns && op.nswork && steps[c].kind != 2 && st > 0;
+            const bool use_ns = allow_ns && op.nswork && steps[c].kind != 2 && st > 0 && ns_step(st);
             (use_ns ? ns : gj).push_back(c);
         }
         SchurParams sp{};
@@ -776,10 +791,8 @@ void factor_engine(Operator &op, hipStream_t s) {
         const int c = std::atoi(cap);
         if (c >= 1 && c <= NS_KMAX) run.update_cap = c;
     }
-    static const bool learn = [] {
-        const char *e = std::getenv("WUST_NS_FIXED_BUDGET");
-        return !(e && e[0] == '1');
-    }();
+    const char *fixed = std::getenv("WUST_NS_FIXED_BUDGET");
+    const bool learn = !(fixed && fixed[0] == '1');
     const BudgetKey key = budget_key(op);
     if (learn && run.allow_ns && op.ns_rounds) {
         std::lock_guard<std::mutex> lk(g_budget_mu);
@@ -808,8 +821,19 @@ void factor_engine(Operator &op, hipStream_t s) {
             need = std::min(need, NS_KMAX);
             next[stp] = (uint8_t)std::max<int>(next[stp], need);
         }
+        if (const char *e = std::getenv("WUST_NS_LOG"); e && e[0] == '1') {
+            float bound;
+            std::memcpy(&bound, st + 4, sizeof(bound));
+            std::fprintf(stderr, "newton-schulz: %d x %d, f %.4g Hz, worst bound %.3g%s; rounds per step:", op.ny,
+                         op.nx, op.ap.f, bound, (st[0] & ST_INVERSE_UNCONVERGED) ? " (unconverged)" : "");
+            for (int j = 0; j < m; ++j)
+                if (j != op.p && used[j] >= 64) std::fprintf(stderr, " [block %d forced at %d]", j, used[j] & 63);
+            for (int i = 0; i < nsteps && i < 48; ++i) std::fprintf(stderr, " %d", (int)next[i]);
+            std::fprintf(stderr, " ...\n");
+        }
         std::lock_guard<std::mutex> lk(g_budget_mu);
-        g_budgets[key] = next;
+        if (st[0] & ST_INVERSE_UNCONVERGED) g_budgets.erase(key);
+        else g_budgets[key] = next;
     }
     op.factor_fallback = 0;
     if (st[0] & ST_INVERSE_UNCONVERGED) {

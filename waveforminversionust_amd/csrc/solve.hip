@@ -15,7 +15,11 @@
 //            (H^H has U_{j-1}^H below and L_{j+1}^H above the diagonal).
 // Layout of every field block: [npad rows][Rpad RHS], RHS fastest, so the
 // coupling loads and the output stores are coalesced across RHS.
+#include <cstdio>
 #include <cstdlib>
+#include <tuple>
+#include <mutex>
+#include <map>
 
 #include "cgemm.h"
 #include "engine.h"
@@ -173,43 +177,48 @@ struct FusedParams {
     int strips, units, per_xcd;
 };
 
-template <int ALAY>
+// KS waves share one 16 x 16 tile (k-blocks dealt round-robin, partial tiles summed in LDS),
+// so a workgroup of four waves covers TPW = 4 / KS adjacent RHS tiles of one strip; DEPTH
+// k-blocks of operands are in flight per wave.  The variant is picked per shape at run time.
+template <int ALAY, int DEPTH, int KS>
 __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
+    constexpr int TPW = 4 / KS;
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
     const int L = xcd * p.per_xcd + local;
     if (L >= p.units) return;
     const int npad = p.npad, Rpad = p.Rpad;
-    const int Nb = Rpad / 16;
-    const int nb = L % Nb;
-    const int mb = (L / Nb) % p.strips;
-    const int task = L / (Nb * p.strips);
+    const int Ng = Rpad / 16 / TPW;
+    const int ng = L % Ng;
+    const int mb = (L / Ng) % p.strips;
+    const int task = L / (Ng * p.strips);
     const FusedTask &tk = p.task[task];
     const int o0 = FUSED_OWN * mb;
     const int o1 = min(o0 + FUSED_OWN - 1, npad - 1);
     const int i0 = min(max(o0 - 1, 0), npad - 16);
-    const int j0 = nb * 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+    const int tw = wave / KS, ks = wave % KS;
+    const int j0 = (ng * TPW + tw) * 16;
 
-    CgemmWave<1, 1, ALAY, B_ROW, false> acc;
+    CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH> acc;
     acc.zero();
     const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
-    acc.run(A, npad, tk.B + j0, Rpad, wave, npad / 16, 4, g, c);
+    acc.run(A, npad, tk.B + j0, Rpad, ks, npad / 16, KS, g, c);
 
-    __shared__ float4 red[3][2][64];
-    __shared__ cf tile[16][17];
-    if (wave > 0) {
-        float re[4], im[4];
-        acc.result(0, 0, re, im);
-        red[wave - 1][0][lane] = make_float4(re[0], re[1], re[2], re[3]);
-        red[wave - 1][1][lane] = make_float4(im[0], im[1], im[2], im[3]);
+    __shared__ float4 red[4][2][64];
+    __shared__ cf tile[TPW][16][17];
+    float re[4], im[4];
+    acc.result(0, 0, re, im);
+    if (KS > 1) {
+        if (ks > 0) {
+            red[wave][0][lane] = make_float4(re[0], re[1], re[2], re[3]);
+            red[wave][1][lane] = make_float4(im[0], im[1], im[2], im[3]);
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (wave == 0) {
-        float re[4], im[4];
-        acc.result(0, 0, re, im);
+    if (ks == 0) {
 #pragma unroll
-        for (int w = 0; w < 3; ++w) {
-            const float4 u = red[w][0][lane], v = red[w][1][lane];
+        for (int w = 1; w < KS; ++w) {
+            const float4 u = red[wave + w][0][lane], v = red[wave + w][1][lane];
             re[0] += u.x; re[1] += u.y; re[2] += u.z; re[3] += u.w;
             im[0] += v.x; im[1] += v.y; im[2] += v.z; im[3] += v.w;
         }
@@ -218,32 +227,120 @@ __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
         for (int r = 0; r < 4; ++r) {
             cf v = mk(re[r], im[r]);
             if (tk.base) v = v + tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
-            tile[4 * g + r][c] = v;
+            tile[tw][4 * g + r][c] = v;
         }
     }
     __syncthreads();
-    // one element per thread: tile row t, column j0 + cc
-    const int t = threadIdx.x >> 4, cc = threadIdx.x & 15;
+    // epilogue, one element per thread and tile: tile row t, column j0' + cc of tile q
+    const int t = (threadIdx.x >> 4) & 15, cc = threadIdx.x & 15;
     const int k = i0 + t;
     if (k < o0 || k > o1) return;
-    const int64_t off = (int64_t)k * Rpad + j0 + cc;
-    const cf v = tile[t][cc];
-    tk.out[off] = v;
-    if (!tk.Bn) return;
-    cf b = tk.nsrc ? tk.nsrc[off] : mk(0.f, 0.f);
-    const cf vn = (k >= 1) ? tile[t - 1][cc] : mk(0.f, 0.f);        // row k - 1 (in the tile when k >= 1)
-    const cf vs = (k + 1 < npad) ? tile[t + 1][cc] : mk(0.f, 0.f);  // row k + 1 (in the tile when < npad)
-    cf a;
-    if (!tk.transposed) {
-        a = tk.cc[k] * v;
-        if (k >= 1) cfma(a, tk.cn[k], vn);
-        if (k + 1 < npad) cfma(a, tk.cs[k], vs);
-    } else {
-        a = conjf_(tk.cc[k]) * v;
-        if (k + 1 < npad) cfma(a, conjf_(tk.cn[k + 1]), vs);
-        if (k >= 1) cfma(a, conjf_(tk.cs[k - 1]), vn);
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+        const int64_t off = (int64_t)k * Rpad + (ng * TPW + q) * 16 + cc;
+        const cf v = tile[q][t][cc];
+        tk.out[off] = v;
+        if (!tk.Bn) continue;
+        cf b = tk.nsrc ? tk.nsrc[off] : mk(0.f, 0.f);
+        const cf vn = (k >= 1) ? tile[q][t - 1][cc] : mk(0.f, 0.f);        // row k - 1 (in the tile when k >= 1)
+        const cf vs = (k + 1 < npad) ? tile[q][t + 1][cc] : mk(0.f, 0.f);  // row k + 1 (in the tile when < npad)
+        cf a;
+        if (!tk.transposed) {
+            a = tk.cc[k] * v;
+            if (k >= 1) cfma(a, tk.cn[k], vn);
+            if (k + 1 < npad) cfma(a, tk.cs[k], vs);
+        } else {
+            a = conjf_(tk.cc[k]) * v;
+            if (k + 1 < npad) cfma(a, conjf_(tk.cn[k + 1]), vs);
+            if (k >= 1) cfma(a, conjf_(tk.cs[k - 1]), vn);
+        }
+        tk.Bn[off] = b - a;
     }
-    tk.Bn[off] = b - a;
+}
+
+// Kernel variants of the fused step and their per-shape choice.  Variant v: (DEPTH, KS) =
+// (2, 4), (3, 4), (4, 2), (2, 2); a variant with TPW tiles per workgroup needs Rpad / 16 to be
+// a multiple of TPW.
+constexpr int N_FUSED_VARIANTS = 4;
+constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2};
+
+template <int ALAY>
+void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool profiled) {
+    const dim3 grid(8 * fp.per_xcd), block(256);
+    auto go = [&](auto kernel) {
+        if (profiled) launch_k(PC_SWEEP_GEMM, kernel, grid, block, 0, s, fp);
+        else {
+            hipLaunchKernelGGL(kernel, grid, block, 0, s, fp);
+            WUST_LAUNCH_CHECK();
+        }
+    };
+    switch (v) {
+        case 1: go(sweep_fused_kernel<ALAY, 3, 4>); break;
+        case 2: go(sweep_fused_kernel<ALAY, 4, 2>); break;
+        case 3: go(sweep_fused_kernel<ALAY, 2, 2>); break;
+        default: go(sweep_fused_kernel<ALAY, 2, 4>); break;
+    }
+}
+
+struct FusedKey {
+    int npad, Rpad, adjoint;
+    bool operator<(const FusedKey &o) const {
+        return std::tie(npad, Rpad, adjoint) < std::tie(o.npad, o.Rpad, o.adjoint);
+    }
+};
+std::mutex g_fused_mu;
+std::map<FusedKey, int> g_fused_choice;
+
+// Times every eligible variant on `fp` (a real step whose outputs the sweep rewrites before
+// reading them) and returns the fastest; the choice is kept per (npad, Rpad, adjoint).
+int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
+    static const bool tune = [] {
+        const char *e = std::getenv("WUST_AUTOTUNE");
+        return !(e && e[0] == '0');
+    }();
+    if (!tune) return 0;
+    const FusedKey key{fp.npad, fp.Rpad, adjoint ? 1 : 0};
+    {
+        std::lock_guard<std::mutex> lk(g_fused_mu);
+        auto it = g_fused_choice.find(key);
+        if (it != g_fused_choice.end()) return it->second;
+    }
+    hipEvent_t e0, e1;
+    WUST_HIP_CHECK(hipEventCreate(&e0));
+    WUST_HIP_CHECK(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < N_FUSED_VARIANTS; ++v) {
+        const int tpw = 4 / FUSED_KS[v];
+        if ((fp.Rpad / 16) % tpw) continue;
+        FusedParams q = fp;
+        q.units = q.ntasks * q.strips * (q.Rpad / 16) / tpw;
+        q.per_xcd = (q.units + 7) / 8;
+        float total = 0.f;
+        for (int rep = 0; rep < 4; ++rep) {
+            WUST_HIP_CHECK(hipEventRecord(e0, s));
+            if (adjoint) launch_fused_variant<A_CONJT>(v, q, s, false);
+            else launch_fused_variant<A_COL>(v, q, s, false);
+            WUST_HIP_CHECK(hipEventRecord(e1, s));
+            WUST_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            WUST_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) total += ms;    // the first launch warms the caches and the code object
+        }
+        if (total < best_ms) {
+            best_ms = total;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (const char *e = std::getenv("WUST_AUTOTUNE_LOG"); e && e[0] == '1')
+        std::fprintf(stderr, "sweep step kernel: npad %d Rpad %d %s -> depth %d, K split %d (%.2f us)\n", fp.npad,
+                     fp.Rpad, adjoint ? "adjoint" : "forward", best == 1 ? 3 : best == 2 ? 4 : 2, FUSED_KS[best],
+                     1e3f * best_ms / 3.f);
+    std::lock_guard<std::mutex> lk(g_fused_mu);
+    g_fused_choice[key] = best;
+    return best;
 }
 
 }  // namespace
@@ -277,6 +374,7 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
         tk.cs = op.coef(dirs[2], cblock);
         tk.transposed = transposed ? 1 : 0;
     };
+    int variant = -1;
     auto launch = [&](std::vector<FusedTask> &tasks) {
         if (tasks.empty()) return;
         FusedParams fp{};
@@ -285,12 +383,12 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
         fp.npad = npad;
         fp.Rpad = Rpad;
         fp.strips = strips;
-        fp.units = fp.ntasks * strips * (Rpad / 16);
+        if (variant < 0) variant = fused_variant(fp, adjoint, s);
+        const int tpw = 4 / FUSED_KS[variant];
+        fp.units = fp.ntasks * strips * (Rpad / 16) / tpw;
         fp.per_xcd = (fp.units + 7) / 8;
-        if (adjoint)
-            launch_k(PC_SWEEP_GEMM, sweep_fused_kernel<A_CONJT>, dim3(8 * fp.per_xcd), dim3(256), 0, s, fp);
-        else
-            launch_k(PC_SWEEP_GEMM, sweep_fused_kernel<A_COL>, dim3(8 * fp.per_xcd), dim3(256), 0, s, fp);
+        if (adjoint) launch_fused_variant<A_CONJT>(variant, fp, s, true);
+        else launch_fused_variant<A_COL>(variant, fp, s, true);
     };
     const int nl = p, nr = m - 1 - p;
     const int nsteps = nl > nr ? nl : nr;
