@@ -64,10 +64,12 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 oracle accuracy / CPU baseline leg")
     ap.add_argument("--cpu-tx", type=int, default=8, help="transmitters in the CPU/accuracy sample")
-    ap.add_argument("--workload", default="ring560", choices=["ring560", "continuation8", "breast1088", "dtft"],
+    ap.add_argument("--workload", default="ring560",
+                    choices=["ring560", "continuation8", "breast1088", "dtft", "cfg1"],
                     help="ring560: the metric case (default); continuation8: eight frequency stages on their own "
                          "grids, one grad eval each; breast1088: a full 36-step reconstruction at 1088^2 x 512 TX; "
-                         "dtft: RF -> frequency DTFT of 256 TX x 256 RX x 4096 float32 samples, 12 frequencies")
+                         "dtft: RF -> frequency DTFT of 256 TX x 256 RX x 4096 float32 samples, 12 frequencies; "
+                         "cfg1: five NCG iterations at 160^2 x 64 TX on the GPU and on the FP64 oracle")
     ap.add_argument("--concurrent", action="store_true",
                     help="continuation8: run the stages on separate streams from host threads")
     return ap.parse_args()
@@ -225,6 +227,65 @@ def run_dtft(args):
     print(json.dumps(line), flush=True)
 
 
+def run_cfg1(args):
+    """The reference's CPU-runnable case: 128^2 interior + 16-cell PML, 64-element ring, two discs,
+    0.5 MHz, 40 dB noise; five NCG iterations from 1500 m/s (mask r < 26 mm, bounds 1400-1600 m/s)."""
+    import numpy as np
+    import torch
+
+    from waveforminversionust_amd import helmholtz as G
+    from waveforminversionust_amd import reconstruct as Rc
+    from waveforminversionust_amd import workloads as W
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    case = W.cpu_reference_case()
+    obs = synthetic_obs(G, case, dev, 1e-12)
+    rng = np.random.default_rng(40)
+    Y = obs["Y"].cpu().numpy()
+    for t in range(case.n_tx):                        # complex Gaussian noise at 40 dB SNR per transmitter
+        sig = np.sqrt(np.mean(np.abs(Y[t, :, 0]) ** 2) / 10 ** 4 / 2)
+        Y[t, :, 0] += sig * (rng.standard_normal(case.n_rx) + 1j * rng.standard_normal(case.n_rx))
+    obs = dict(obs, Y=Y, mask=np.ones(Y.shape, bool))
+    X, Yg = np.meshgrid(case.x_m, case.y_m)
+    mask = X ** 2 + Yg ** 2 < 26e-3 ** 2
+    cfg = dict(case.cfg, step_damping=0.25, max_update_mps=12.0, bounds_mps=[1400.0, 1600.0], filter_cutoff=0.0,
+               filter_order=4, update_mask=mask, frequency_schedule_hz=[0.5e6] * 5, tol=args.tol)
+    init = 1500 * np.ones((case.ny, case.nx))
+    for _ in range(args.warmup):
+        Rc.wust_reconstruct(obs, init, dict(cfg, device=dev))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = Rc.wust_reconstruct(obs, init, dict(cfg, device=dev))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    c = res["c_mps"].cpu().numpy()
+    err = lambda m: float(np.linalg.norm((m - case.c_true)[mask]) / np.linalg.norm(case.c_true[mask]))
+    cpu = None
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+
+        from oracle import helmholtz_fp64 as H
+        from oracle import ncg_fp64 as O
+        with threadpool_limits(1):
+            c0 = time.perf_counter()
+            co, _ = O.wust_reconstruct(obs, init, dict(cfg, schedule=[0] * 5), H.wust_oracle,
+                                       H.directional_curvature)
+            c_s = time.perf_counter() - c0
+        cpu = dict(value=round(c_s, 2), unit="s", cores=1, kind="port",
+                   sample=f"FP64 oracle loop (scipy SuperLU, 1 thread), the same 5 NCG iterations: {c_s:.2f} s; "
+                          f"final model rel err {err(co):.4%}, max |c_gpu - c_cpu| {np.max(np.abs(c - co)):.2e} m/s")
+    line = {
+        "metric": "cfg1 reconstruction wall time (160^2, 64 TX, 5 NCG iterations at 0.5 MHz)",
+        "value": round(elapsed, 4), "unit": "s", "n_gpus": 1, "steps": 5, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / 5, 2), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (fp32 MFMA) + f64 refinement", "data": "synthetic",
+        "config": {"workload": case.name, "n_tx": case.n_tx, "refine_tol": args.tol, "snr_db": 40},
+        "model_rel_err_initial_final": [round(err(init), 6), round(err(c), 6)],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_breast(args):
     """Full reconstruction: 12 frequencies 0.3-1.2 MHz x 3 NCG iterations on 1088^2 nodes, 512 TX."""
     import numpy as np
@@ -284,6 +345,8 @@ def main():
         return run_breast(args)
     if args.workload == "dtft":
         return run_dtft(args)
+    if args.workload == "cfg1":
+        return run_cfg1(args)
     import numpy as np
     import torch
 

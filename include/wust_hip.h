@@ -124,6 +124,11 @@ int wust_stencil_opt_params(double vmin, double vmax, double f, double h, double
 int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const double *d_atten,
                           int mode, void *stream, wust_helmholtz **out);
 int wust_helmholtz_destroy(wust_helmholtz *h);
+
+/* Device memory of destroyed operators is kept (up to WUST_POOL_MB megabytes, default 98304)
+ * for the next operator with the same sizes; wust_pool_trim frees all of it.  No reference
+ * counterpart (the MEX gateways allocate and free per call). */
+int wust_pool_trim(void);
 int wust_helmholtz_info(const wust_helmholtz *h, wust_info *info);
 
 /* The nine coefficient arrays of ninePointCoefficients in the reference

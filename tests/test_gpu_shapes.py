@@ -9,6 +9,7 @@ and adjoint) and the loss / gradient of one oracle evaluation.
 """
 import numpy as np
 import pytest
+import torch
 
 from oracle import helmholtz_fp64 as H
 
@@ -76,3 +77,22 @@ def test_oracle_evaluation_on_ragged_grids(ny, nx):
     lg, gg = G.wust_oracle(init, obs, 0, cfg)
     assert abs(lg - lo) <= 1e-9 * abs(lo)
     assert np.linalg.norm(gg.cpu().numpy() - go) <= 1e-8 * np.linalg.norm(go)
+
+
+def test_pooled_buffers_reproduce_results():
+    """An operator built on buffers of a destroyed one (the device pool) gives bit-identical results."""
+    from waveforminversionust_amd import _native as N
+    x, y, c = _grid(40, 23)
+    f, a0, L = 2.2e5, 10.0, 2e-3
+    rng = np.random.default_rng(3)
+    b = np.zeros((40, 23, 7), np.complex128)
+    b[1:-1, 1:-1] = rng.standard_normal((38, 21, 7)) + 1j * rng.standard_normal((38, 21, 7))
+    N.check(N.load().wust_pool_trim())
+    results = []
+    for _ in range(3):
+        s = G.HelmholtzSolver(x, y, c, None, f, -1, a0, L, dict(OPTS, mode="refined", tol=1e-13))
+        results.append(s.solve(b, False)[0].clone())
+        s.close()
+        if len(results) == 2:
+            N.check(N.load().wust_pool_trim())
+    assert torch.equal(results[0], results[1]) and torch.equal(results[0], results[2])

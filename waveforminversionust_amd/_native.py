@@ -69,6 +69,7 @@ EXPORTS = {
     "wust_helmholtz_create": (C.c_int, [C.POINTER(GridDesc), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                         C.POINTER(C.c_void_p)]),
     "wust_helmholtz_destroy": (C.c_int, [C.c_void_p]),
+    "wust_pool_trim": (C.c_int, []),
     "wust_helmholtz_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
     "wust_helmholtz_coefficients": (C.c_int, [C.c_void_p] * 11),
     "wust_helmholtz_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_double,

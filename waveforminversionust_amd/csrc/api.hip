@@ -1,5 +1,8 @@
 // C ABI of the engine (declared in include/wust_hip.h): handle lifetime,
 // validation, refinement driver, oracle (loss/gradient) and curvature drivers.
+#include <cstdlib>
+#include <mutex>
+#include <map>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -40,19 +43,73 @@ int guarded(F &&fn) {
     }
 }
 
+// Device buffers of destroyed operators are kept for the next operator with the same sizes
+// (one operator is typically created per model update, with identical shapes): the
+// destructor synchronizes the device once and hands its buffers to a free list keyed by
+// (device, bytes); an allocation of that size takes one back instead of calling hipMalloc.
+// Only the destructor pools buffers, so none can still be in use by queued kernels.  The
+// pool holds at most WUST_POOL_MB megabytes (default 98304); wust_pool_trim() frees it.
+struct DevicePool {
+    std::mutex mu;
+    std::map<std::pair<int, size_t>, std::vector<void *>> free_list;
+    size_t held = 0;
+    size_t cap = [] {
+        const char *e = std::getenv("WUST_POOL_MB");
+        const long long mb = e ? std::atoll(e) : 98304;
+        return (size_t)(mb > 0 ? mb : 0) << 20;
+    }();
+};
+DevicePool &pool() {
+    static DevicePool *p = new DevicePool();   // never destroyed: buffers outlive static teardown
+    return *p;
+}
+int current_device() {
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return d;
+}
+
 template <typename T>
 T *dalloc(int64_t count) {
     void *p = nullptr;
     if (count <= 0) return nullptr;
-    hipError_t e = hipMalloc(&p, sizeof(T) * (size_t)count);
+    const size_t bytes = sizeof(T) * (size_t)count;
+    {
+        DevicePool &pl = pool();
+        std::lock_guard<std::mutex> lk(pl.mu);
+        auto it = pl.free_list.find({current_device(), bytes});
+        if (it != pl.free_list.end() && !it->second.empty()) {
+            p = it->second.back();
+            it->second.pop_back();
+            pl.held -= bytes;
+            return static_cast<T *>(p);
+        }
+    }
+    hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess)
-        throw Error(WUST_HIP_ERROR, std::string("hipMalloc(") + std::to_string(sizeof(T) * count) +
-                                         " bytes): " + hipGetErrorString(e));
+        throw Error(WUST_HIP_ERROR, std::string("hipMalloc(") + std::to_string(bytes) + " bytes): " +
+                                         hipGetErrorString(e));
     return static_cast<T *>(p);
 }
+// immediate release (hipFree synchronizes with all queued work)
 template <typename T>
 void dfree(T *&p) {
     if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+// release into the pool; the caller has synchronized the device
+template <typename T>
+void dpool(T *&p, int64_t count) {
+    if (!p) return;
+    const size_t bytes = sizeof(T) * (size_t)count;
+    DevicePool &pl = pool();
+    std::lock_guard<std::mutex> lk(pl.mu);
+    if (count > 0 && pl.held + bytes <= pl.cap) {
+        pl.free_list[{current_device(), bytes}].push_back((void *)p);
+        pl.held += bytes;
+    } else {
+        (void)hipFree((void *)p);
+    }
     p = nullptr;
 }
 
@@ -135,6 +192,18 @@ struct wust_helmholtz {
 
     ~wust_helmholtz() {
         release_retained();
+        if (hipDeviceSynchronize() == hipSuccess) {
+            // big buffers go back to the pool for the next operator of this shape
+            const int64_t nn = (int64_t)op.ny * op.nx, pg = (int64_t)op.npad * op.npad;
+            pool_ws();
+            dpool(op.ck2, nn);
+            dpool(op.vsm, nn);
+            dpool(op.w64, 9 * (int64_t)op.m * op.n);
+            dpool(op.w32, 9 * (int64_t)op.m * op.npad);
+            dpool(op.pages, pg * op.m);
+            dpool(op.work, 4 * pg);
+            if (op.nswork) dpool(op.nswork, 6 * pg);
+        }
         release_ws();
         dfree(op.ck2);
         dfree(op.vsm);
@@ -148,6 +217,18 @@ struct wust_helmholtz {
         dfree(op.ns_rounds);
         dfree(op.status);
         dfree(idx);
+    }
+    // the solve workspace of Rpad RHS into the pool (same sizes as ensure_ws allocates)
+    void pool_ws() {
+        if (!ws_Rpad) return;
+        const int64_t cnt = field_elems(ws_Rpad);
+        dpool(b64, cnt);
+        dpool(t64, cnt);
+        dpool(u64, cnt);
+        dpool(lam64, cnt);
+        if (s32) dpool(s32, cnt);
+        if (d32) dpool(d32, cnt);
+        if (b32) dpool(b32, solve_work_elems(op, ws_Rpad));
     }
     void release_retained() {
         for (auto &r : retained) {
@@ -446,6 +527,21 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
 
 int wust_helmholtz_destroy(wust_helmholtz *h) {
     return guarded([&] { delete h; });
+}
+
+int wust_pool_trim(void) {
+    return guarded([&] {
+        DevicePool &pl = pool();
+        std::lock_guard<std::mutex> lk(pl.mu);
+        int dev0 = current_device();
+        for (auto &kv : pl.free_list) {
+            WUST_HIP_CHECK(hipSetDevice(kv.first.first));
+            for (void *p : kv.second) (void)hipFree(p);
+        }
+        WUST_HIP_CHECK(hipSetDevice(dev0));
+        pl.free_list.clear();
+        pl.held = 0;
+    });
 }
 
 int wust_helmholtz_info(const wust_helmholtz *h, wust_info *info) {
