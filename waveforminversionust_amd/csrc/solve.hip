@@ -199,6 +199,33 @@ __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
     const int tw = wave / KS, ks = wave % KS;
     const int j0 = (ng * TPW + tw) * 16;
 
+    // epilogue operands (base rows of this lane's tile, and for this thread's output element the
+    // next source and the three coupling coefficients) are loaded before the GEMM so that their
+    // latency overlaps it instead of trailing it
+    const int t = (threadIdx.x >> 4) & 15, cc = threadIdx.x & 15;
+    const int k = i0 + t;
+    const bool own = k >= o0 && k <= o1;
+    cf pbase[4];
+    if (tk.base && ks == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pbase[r] = tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
+    }
+    cf pn = mk(0.f, 0.f), pc = mk(0.f, 0.f), ps = mk(0.f, 0.f), psrc[TPW];
+    if (own && tk.Bn) {
+        if (!tk.transposed) {
+            pc = tk.cc[k];
+            if (k >= 1) pn = tk.cn[k];
+            if (k + 1 < npad) ps = tk.cs[k];
+        } else {
+            pc = conjf_(tk.cc[k]);
+            if (k >= 1) pn = conjf_(tk.cs[k - 1]);
+            if (k + 1 < npad) ps = conjf_(tk.cn[k + 1]);
+        }
+#pragma unroll
+        for (int q = 0; q < TPW; ++q)
+            psrc[q] = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + (ng * TPW + q) * 16 + cc] : mk(0.f, 0.f);
+    }
+
     CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH> acc;
     acc.zero();
     const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
@@ -226,35 +253,32 @@ __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             cf v = mk(re[r], im[r]);
-            if (tk.base) v = v + tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
+            if (tk.base) v = v + pbase[r];
             tile[tw][4 * g + r][c] = v;
         }
     }
     __syncthreads();
     // epilogue, one element per thread and tile: tile row t, column j0' + cc of tile q
-    const int t = (threadIdx.x >> 4) & 15, cc = threadIdx.x & 15;
-    const int k = i0 + t;
-    if (k < o0 || k > o1) return;
+    if (!own) return;
 #pragma unroll
     for (int q = 0; q < TPW; ++q) {
         const int64_t off = (int64_t)k * Rpad + (ng * TPW + q) * 16 + cc;
         const cf v = tile[q][t][cc];
         tk.out[off] = v;
         if (!tk.Bn) continue;
-        cf b = tk.nsrc ? tk.nsrc[off] : mk(0.f, 0.f);
         const cf vn = (k >= 1) ? tile[q][t - 1][cc] : mk(0.f, 0.f);        // row k - 1 (in the tile when k >= 1)
         const cf vs = (k + 1 < npad) ? tile[q][t + 1][cc] : mk(0.f, 0.f);  // row k + 1 (in the tile when < npad)
-        cf a;
+        // the same products and order as operand_b: P v = c_c v_k + c_n v_{k-1} + c_s v_{k+1}
+        // (transposed: conj(cc_k) v_k + conj(cn_{k+1}) v_{k+1} + conj(cs_{k-1}) v_{k-1})
+        cf a = pc * v;
         if (!tk.transposed) {
-            a = tk.cc[k] * v;
-            if (k >= 1) cfma(a, tk.cn[k], vn);
-            if (k + 1 < npad) cfma(a, tk.cs[k], vs);
+            if (k >= 1) cfma(a, pn, vn);
+            if (k + 1 < npad) cfma(a, ps, vs);
         } else {
-            a = conjf_(tk.cc[k]) * v;
-            if (k + 1 < npad) cfma(a, conjf_(tk.cn[k + 1]), vs);
-            if (k >= 1) cfma(a, conjf_(tk.cs[k - 1]), vn);
+            if (k + 1 < npad) cfma(a, ps, vs);
+            if (k >= 1) cfma(a, pn, vn);
         }
-        tk.Bn[off] = b - a;
+        tk.Bn[off] = psrc[q] - a;
     }
 }
 
