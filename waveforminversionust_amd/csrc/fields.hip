@@ -89,15 +89,36 @@ __constant__ int c_dy[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
 __constant__ int c_dx[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
 __constant__ int c_opp[9] = {DC, DS, DN, DE, DSE, DNE, DW, DSW, DNW};
 
-// One thread per (RHS r, row y, column x): the nine neighbours are
-// independent loads (more memory-level parallelism than a register window
-// marching along x, which measured slower), fused u += d, per-RHS |r|^2 and
-// |b|^2 reduced per workgroup in LDS before the atomics.
+// One thread per (RHS r, row y, column x), a workgroup per 16 RHS x 16 rows of
+// one column.  The workgroup's 9 x 16 coupling coefficients (the same for all
+// RHS) are staged once in LDS, zero for couplings that leave the interior;
+// the nine neighbour reads then go to clamped in-range nodes and are issued
+// unconditionally, so all of them are in flight at once.  A clamped node lies
+// inside the thread's own stencil, and its zero coefficient adds an exact
+// zero.  Fused u += d; per-RHS |r|^2 and |b|^2 reduced per workgroup in LDS
+// before the atomics.
+template <bool HAS_U, bool HAS_D>
 __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
-    const int r = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int yi = blockIdx.y * 16 + (threadIdx.x >> 4);
+    constexpr int DY[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
+    constexpr int DX[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
+    const int lr = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int r = blockIdx.x * 16 + lr;
+    const int yi = blockIdx.y * 16 + ly;
     const int xi = blockIdx.z;
     const int n = a.n, m = a.m, npad = a.npad, Rpad = a.Rpad;
+    __shared__ cd wt[9][16];
+    if (threadIdx.x < 144) {
+        const int dd = threadIdx.x >> 4, ty = blockIdx.y * 16 + (threadIdx.x & 15);
+        const int yy = ty + c_dy[dd], xx = xi + c_dx[dd];
+        const int64_t plane = (int64_t)m * n;
+        cd w = mkd(0, 0);
+        if (ty < n && yy >= 0 && yy < n && xx >= 0 && xx < m) {
+            if (!a.adjoint) w = a.w64[dd * plane + (int64_t)xi * n + ty];
+            else w = conjd(a.w64[c_opp[dd] * plane + (int64_t)xx * n + yy]);
+        }
+        wt[dd][threadIdx.x & 15] = w;
+    }
+    __syncthreads();
     double contrib = 0, bcontrib = 0;
     if (yi < npad && r < Rpad) {
         const int64_t o = ((int64_t)xi * npad + yi) * Rpad + r;
@@ -106,24 +127,23 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
             if (a.res32) a.res32[o] = mk(0.f, 0.f);
             if (a.res64) a.res64[o] = mkd(0, 0);
         } else {
-            const int64_t plane = (int64_t)m * n;
-            cd hu = mkd(0, 0);
-            cd self = mkd(0, 0);
+            const int ys[3] = {yi, max(yi - 1, 0), min(yi + 1, n - 1)};
+            const int xs[3] = {xi, max(xi - 1, 0), min(xi + 1, m - 1)};
+            cd u[9];
 #pragma unroll
             for (int dd = 0; dd < 9; ++dd) {
-                int yy = yi + c_dy[dd], xx = xi + c_dx[dd];
-                if (yy < 0 || yy >= n || xx < 0 || xx >= m) continue;
-                int64_t on = ((int64_t)xx * npad + yy) * Rpad + r;
-                cd u = mkd(0, 0);
-                if (a.u_old) u = a.u_old[on];
-                if (a.d32) u = u + tod(a.d32[on]);
-                if (dd == DC) self = u;
-                cd w;
-                if (!a.adjoint) w = a.w64[dd * plane + (int64_t)xi * n + yi];
-                else w = conjd(a.w64[c_opp[dd] * plane + (int64_t)xx * n + yy]);
-                hu = hu + w * u;
+                const int yy = ys[DY[dd] < 0 ? 1 : (DY[dd] > 0 ? 2 : 0)];
+                const int xx = xs[DX[dd] < 0 ? 1 : (DX[dd] > 0 ? 2 : 0)];
+                const int64_t on = ((int64_t)xx * npad + yy) * Rpad + r;
+                u[dd] = mkd(0, 0);
+                if (HAS_U) u[dd] = a.u_old[on];
+                if (HAS_D) u[dd] = u[dd] + tod(a.d32[on]);
             }
-            cd b = a.b64 ? a.b64[o] : mkd(0, 0);
+            const cd b = a.b64 ? a.b64[o] : mkd(0, 0);
+            cd hu = mkd(0, 0);
+#pragma unroll
+            for (int dd = 0; dd < 9; ++dd) hu = hu + wt[dd][ly] * u[dd];
+            const cd self = u[DC];
             cd res = b - hu;
             if (a.u_new) a.u_new[o] = self;
             if (a.res32) a.res32[o] = tof(res);
@@ -354,7 +374,10 @@ void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int 
 }
 void launch_residual(const ResidualArgs &a, hipStream_t s) {
     dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);
-    launch_k(PC_RESIDUAL, residual_kernel, grid, dim3(256), 0, s, a);
+    if (a.u_old && a.d32) launch_k(PC_RESIDUAL, residual_kernel<true, true>, grid, dim3(256), 0, s, a);
+    else if (a.u_old) launch_k(PC_RESIDUAL, residual_kernel<true, false>, grid, dim3(256), 0, s, a);
+    else if (a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, true>, grid, dim3(256), 0, s, a);
+    else launch_k(PC_RESIDUAL, residual_kernel<false, false>, grid, dim3(256), 0, s, a);
 }
 void promote(const cf *in, cd *out, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(promote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
