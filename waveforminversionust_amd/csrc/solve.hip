@@ -15,6 +15,7 @@
 //            (H^H has U_{j-1}^H below and L_{j+1}^H above the diagonal).
 // Layout of every field block: [npad rows][Rpad RHS], RHS fastest, so the
 // coupling loads and the output stores are coalesced across RHS.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <tuple>
@@ -282,6 +283,20 @@ __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
     }
 }
 
+// Zeroes rows r0 .. npad-1 of the [npad][Rpad] blocks of two arrays (blockIdx.y selects the
+// array): the padding rows that the fused steps' strips do not cover, which the steps read as
+// operand rows (against zero columns of the pages) and as tile halos (against zero couplings),
+// so they must hold finite values -- zeros, as every other padding row does.
+__global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r0, int npad, int Rpad) {
+    cf *dst = blockIdx.y ? b : a;
+    const int64_t count = blockIdx.y ? nb : na;
+    const int64_t per = (int64_t)(npad - r0) * Rpad;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= per * count) return;
+    const int64_t blkno = i / per, e = i % per;
+    dst[blkno * npad * Rpad + (int64_t)r0 * Rpad + e] = mk(0.f, 0.f);
+}
+
 // Kernel variants of the fused step and their per-shape choice.  Variant v: (DEPTH, KS) =
 // (2, 4), (3, 4), (4, 2), (2, 2); a variant with TPW tiles per workgroup needs Rpad / 16 to be
 // a multiple of TPW.
@@ -391,7 +406,16 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
     auto OP = [&](int chain, int pp) { return ops + (int64_t)(2 * chain + pp) * blk; };
     int Ld[3] = {DNW, DW, DSW};
     int Ud[3] = {DNE, DE, DSE};
-    const int strips = (npad + FUSED_OWN - 1) / FUSED_OWN;
+    // strips cover the n real rows only (1280 rather than 1344 workgroups at 558 rows and 256 RHS:
+    // five per CU); the padding rows beyond them are zeroed once per solve
+    const int strips = (op.n + FUSED_OWN - 1) / FUSED_OWN;
+    const int covered = std::min(strips * FUSED_OWN, npad);
+    if (covered < npad) {
+        const int64_t per = (int64_t)(npad - covered) * Rpad, most = per * (m + 4);
+        hipLaunchKernelGGL(pad_rows_zero_kernel, dim3((unsigned)((most + 255) / 256), 2), dim3(256), 0, s, work,
+                           (int64_t)m + 4, out, (int64_t)m, covered, npad, Rpad);
+        WUST_LAUNCH_CHECK();
+    }
     auto couple = [&](FusedTask &tk, const int dirs[3], int cblock, bool transposed) {
         tk.cn = op.coef(dirs[0], cblock);
         tk.cc = op.coef(dirs[1], cblock);
