@@ -80,21 +80,29 @@ __global__ void __launch_bounds__(256) schur_kernel(SchurParams p) {
     if (r == c) val = tk.Dc[r];
     else if (c == r - 1) val = tk.Dn[r];
     else if (c == r + 1) val = tk.Ds[r];
+    // the nine X entries come from clamped in-range indices with zero coefficients where the
+    // stencil leaves the block, so all loads issue at once; a zero coefficient adds an exact zero
+    const int rows[3] = {max(r - 1, 0), r, min(r + 1, n - 1)};
+    const int64_t cols[3] = {(int64_t)ld * c, (int64_t)ld * max(c - 1, 0), (int64_t)ld * min(c + 1, n - 1)};
+    const cf zero = mk(0.f, 0.f);
     for (int t = 0; t < tk.nterms; ++t) {
         const SchurTerm &st = tk.t[t];
-        cf qm = (c >= 1) ? st.Qs[c - 1] : mk(0.f, 0.f);
-        cf q0 = st.Qc[c];
-        cf qp = (c + 1 < n) ? st.Qn[c + 1] : mk(0.f, 0.f);
-        cf acc = mk(0.f, 0.f);
-        for (int da = -1; da <= 1; ++da) {
-            int a = r + da;
-            if (a < 0 || a >= n) continue;
-            cf pr = (da < 0) ? st.Pn[r] : (da == 0 ? st.Pc[r] : st.Ps[r]);
-            const cf *Xa = st.X + a;
-            cf xq = q0 * Xa[(int64_t)ld * c];
-            if (c >= 1) cfma(xq, qm, Xa[(int64_t)ld * (c - 1)]);
-            if (c + 1 < n) cfma(xq, qp, Xa[(int64_t)ld * (c + 1)]);
-            cfma(acc, pr, xq);
+        const cf qm = (c >= 1) ? st.Qs[c - 1] : zero;
+        const cf q0 = st.Qc[c];
+        const cf qp = (c + 1 < n) ? st.Qn[c + 1] : zero;
+        const cf pr[3] = {(r >= 1) ? st.Pn[r] : zero, st.Pc[r], (r + 1 < n) ? st.Ps[r] : zero};
+        cf x[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) x[a][q] = st.X[rows[a] + cols[q]];
+        cf acc = zero;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            cf xq = q0 * x[a][0];
+            cfma(xq, qm, x[a][1]);
+            cfma(xq, qp, x[a][2]);
+            cfma(acc, pr[a], xq);
         }
         val = val - acc;
     }
