@@ -336,8 +336,9 @@ def run_breast(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the transmitter-sharded multi-GPU layout is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: a multi-GPU run would "
+              "replicate the factor on every GPU, shard the transmitters and all-reduce the gradient, "
+              "which is not built", file=sys.stderr)
         sys.exit(2)
     if args.workload == "continuation8":
         return run_continuation(args)
