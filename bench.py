@@ -418,8 +418,9 @@ def main():
         loss_ref, g_ref = step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    cnt, ms = N.C.c_int64(), N.C.c_double()
+    cnt, ms, units = N.C.c_int64(), N.C.c_double(), N.C.c_int64()
     lib.wust_profile_read(dom_idx, N.C.byref(cnt), N.C.byref(ms))
+    lib.wust_profile_units(dom_idx, N.C.byref(units))
     lib.wust_profile_enable(0)
     value = args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
@@ -428,11 +429,12 @@ def main():
     # roofline of the dominant kernel (algorithmic flops; n = Ny-2 interior rows per block)
     n, m, R = ny - 2, nx - 2, ntx
     # each launch = up to two chain steps, each an n x n times n x R complex product (8 flops / MAC);
-    # a solve = 2m-1 chain steps in m+1 launches (twisted sweeps)
-    solves = round(cnt.value / (m + 1))   # solves in the profiled step
-    total_flops = 8.0 * n * n * R * (2 * m - 1) * solves
+    # the library counts the chain steps its profiled launches executed (a solve has 2m-1 in m+1
+    # launches, fewer when the forward elimination skips columns where the right-hand side is zero)
+    total_flops = 8.0 * n * n * R * units.value
     flops_per_launch = total_flops / max(cnt.value, 1)
-    per_unit = "8*n^2*R flops per chain step (n=558, R=256); 2m-1 chain steps per solve in m+1 launches"
+    per_unit = (f"8*n^2*R flops per chain step (n={n}, R={R}); {units.value} chain steps executed in "
+                f"{cnt.value} launches (2m-1 = {2 * m - 1} per full solve in m+1 = {m + 1} launches)")
     avg_launch_s = ms.value / 1e3 / max(cnt.value, 1)
     achieved = (total_flops / (ms.value / 1e3)) / 1e12 if ms.value > 0 else 0.0
     roofline = dict(kernel=dominant, bound="mfma", achieved=round(achieved, 2), peak=MFMA_FP32_PEAK_TFLOPS,

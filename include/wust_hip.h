@@ -250,6 +250,10 @@ int wust_rf_dtft(const void *d_rf, int rf_dtype, int64_t n_time, int32_t n_rx, i
 int wust_profile_enable(uint32_t class_mask);
 int wust_profile_read(int kernel_class, int64_t *count, double *total_ms);
 int wust_profile_reset(void);
+/* Work units counted with the profiled launches of a class since the last reset: for the sweep
+ * GEMM class, block chain steps (one n x n by n x R complex product each), so that a caller can
+ * price the class's time against the work actually executed. */
+int wust_profile_units(int kernel_class, int64_t *units);
 
 #ifdef __cplusplus
 }

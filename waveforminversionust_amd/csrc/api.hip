@@ -215,6 +215,7 @@ struct wust_helmholtz {
         dfree(op.nswork);
         dfree(op.ns_norm);
         dfree(op.ns_rounds);
+        dfree(op.colflags);
         dfree(op.status);
         dfree(idx);
     }
@@ -290,6 +291,25 @@ struct wust_helmholtz {
     }
     void clear_status(hipStream_t s) { WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, sizeof(uint32_t), s)); }
 
+    // b -> s32, and the numbers of leading and trailing block columns of b that are zero (point
+    // sources and receivers leave the columns between the array and the domain edges empty; the
+    // sweeps skip their forward elimination)
+    void demote_rhs(const cd *b, int64_t cnt, int Rpad, hipStream_t s, int &lead, int &trail) {
+        lead = trail = 0;
+        const char *e = std::getenv("WUST_SKIP_ZERO_COLUMNS");
+        if (!op.colflags || (e && e[0] == '0')) {
+            demote(b, s32, cnt, s);
+            return;
+        }
+        WUST_HIP_CHECK(hipMemsetAsync(op.colflags, 0, sizeof(int) * op.m, s));
+        demote_flag(b, s32, cnt, (int64_t)op.npad * Rpad, op.colflags, s);
+        std::vector<int> f((size_t)op.m);
+        WUST_HIP_CHECK(hipMemcpyAsync(f.data(), op.colflags, sizeof(int) * op.m, hipMemcpyDeviceToHost, s));
+        WUST_HIP_CHECK(hipStreamSynchronize(s));
+        while (lead < op.m && !f[lead]) ++lead;
+        while (trail < op.m && !f[op.m - 1 - trail]) ++trail;
+    }
+
     // Solve in the solve layout: b (c128, Rpad) -> out (c128).  b is not modified.
     wust_solve_stats solve_internal(const cd *b, cd *out, int R, int Rpad, bool adjoint, double tol, int max_iter,
                                     bool want_residual, hipStream_t s) {
@@ -335,8 +355,9 @@ struct wust_helmholtz {
             }
             return st;
         }
-        demote(b, s32, cnt, s);
-        solve_engine(op, s32, d32, b32, Rpad, adjoint, s);
+        int lead = 0, trail = 0;
+        demote_rhs(b, cnt, Rpad, s, lead, trail);
+        solve_engine(op, s32, d32, b32, Rpad, adjoint, s, lead, trail);
         if (op.mode == WUST_MODE_C64) {
             promote(d32, out, cnt, s);
             if (want_residual) st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr);
@@ -491,6 +512,7 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
             op.ns_norm = dalloc<float>(2 * 8);
             op.ns_rounds = dalloc<int>(op.m);
         }
+        if (mode != WUST_MODE_PLAIN) op.colflags = dalloc<int>(op.m);
         op.factor_bytes = (int64_t)sizeof(cf) * pg * op.m;
         WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, 8 * sizeof(uint32_t), s));
 

@@ -32,6 +32,7 @@ struct Operator {
     cf *nswork = nullptr;         // Newton-Schulz iterates and cubic operand: [2 chains][3][npad*npad]
     float *ns_norm = nullptr;     // ||E_r||_F^2 per chain and round: [2 chains][8]
     int *ns_rounds = nullptr;     // Newton-Schulz rounds used per block, + 64 if the last was forced: [m]
+    int *colflags = nullptr;      // per block column: 1 when a right-hand side has a nonzero there: [m]
     uint32_t *status = nullptr;   // device status words: [0] flags, [2..3] pivot ratios (float),
                                   // [4] largest Newton-Schulz residual bound (float)
     double assembly_ms = 0, factor_ms = 0, min_pivot_ratio = 1.0;
@@ -53,7 +54,11 @@ void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s);
 // src, out: [m][npad][Rpad] complex-FP32; out must not alias src; scratch: 2*npad*Rpad.
 // Rpad must be a multiple of 16.
 // work: [m][npad][Rpad] forward iterates + 4 operand blocks [npad][Rpad] (solve_work_elems)
-void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad, bool adjoint, hipStream_t s);
+// lead / trail: block columns 0 .. lead-1 and m-trail .. m-1 of src are zero, so the forward
+// elimination of each chain starts at its first nonzero column (its iterates before it are
+// exactly zero); each chain still computes at least its last step.
+void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad, bool adjoint, hipStream_t s,
+                  int lead = 0, int trail = 0);
 inline int64_t solve_work_elems(const Operator &op, int Rpad) { return ((int64_t)op.m + 4) * op.npad * Rpad; }
 
 }  // namespace wust

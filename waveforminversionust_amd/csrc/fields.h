@@ -64,6 +64,9 @@ void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int 
 void launch_residual(const ResidualArgs &a, hipStream_t s);
 void promote(const cf *in, cd *out, int64_t count, hipStream_t s);
 void demote(const cd *in, cf *out, int64_t count, hipStream_t s);
+// demote, and flags[i / per] = 1 for every block of `per` consecutive values that holds a nonzero
+// (flags pre-zeroed; per a multiple of 64)
+void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, hipStream_t s);
 void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s);
 void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s);
 void launch_gradient(const GradientArgs &a, hipStream_t s);

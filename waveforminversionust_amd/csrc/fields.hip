@@ -176,6 +176,18 @@ __global__ void demote_kernel(const cd *__restrict__ in, cf *__restrict__ out, i
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < count) out[i] = tof(in[i]);
 }
+// A wave's 64 values lie in one block of `per` (a multiple of 64), so one lane stores the flag.
+__global__ void demote_flag_kernel(const cd *__restrict__ in, cf *__restrict__ out, int64_t count, int64_t per,
+                                   int *__restrict__ flags) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool nz = false;
+    if (i < count) {
+        const cd v = in[i];
+        out[i] = tof(v);
+        nz = (v.re != 0.0) || (v.im != 0.0);
+    }
+    if (__any(nz) && (threadIdx.x & 63) == 0) flags[i / per] = 1;
+}
 
 // unit impulses: B[xi][yi][r] = 1 at the source node of RHS r (B pre-zeroed)
 __global__ void impulse_kernel(cd *__restrict__ b, const int *__restrict__ sx, const int *__restrict__ sy, int R,
@@ -381,6 +393,11 @@ void launch_residual(const ResidualArgs &a, hipStream_t s) {
 }
 void promote(const cf *in, cd *out, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(promote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
+    WUST_LAUNCH_CHECK();
+}
+void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, hipStream_t s) {
+    hipLaunchKernelGGL(demote_flag_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count,
+                       per, flags);
     WUST_LAUNCH_CHECK();
 }
 void demote(const cd *in, cf *out, int64_t count, hipStream_t s) {

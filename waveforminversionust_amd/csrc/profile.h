@@ -26,6 +26,8 @@ void prof_record(int cls, bool begin, hipStream_t s);
 void prof_flush();
 hipEvent_t prof_event();
 void prof_push(int cls, hipEvent_t a, hipEvent_t b);
+// count n work units (the class's own unit, e.g. chain steps of a sweep launch) when cls is profiled
+void prof_units(int cls, int64_t n);
 
 // Launch a kernel; when its class is profiled, the dispatch packet itself
 // timestamps start and end (hipExtLaunchKernelGGL events), so the timing adds

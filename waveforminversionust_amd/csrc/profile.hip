@@ -26,6 +26,7 @@ struct Profiler {
     std::vector<hipEvent_t> open;   // begin events waiting for their end, per class
     double total_ms[PC_COUNT] = {};
     int64_t count[PC_COUNT] = {};
+    int64_t units[PC_COUNT] = {};   // work units of the profiled launches (per class; see prof_units)
     Profiler() : open(PC_COUNT, nullptr) {}
     // resolve one pending pair (blocking when `wait`); false if it has not completed
     bool resolve_front(bool wait) {
@@ -98,6 +99,13 @@ void prof_push(int cls, hipEvent_t a, hipEvent_t b) {
     p.pending.push_back({cls, a, b});
 }
 
+void prof_units(int cls, int64_t n) {
+    if (!prof_on(cls)) return;
+    Profiler &p = P();
+    std::lock_guard<std::mutex> lk(p.mu);
+    p.units[cls] += n;
+}
+
 void prof_flush() {
     Profiler &p = P();
     std::lock_guard<std::mutex> lk(p.mu);
@@ -132,6 +140,13 @@ int wust_profile_read(int cls, int64_t *count, double *total_ms) {
     return WUST_OK;
 }
 
+int wust_profile_units(int cls, int64_t *units) {
+    if (cls < 0 || cls >= wust::PC_COUNT || !units) return WUST_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(wust::P().mu);
+    *units = wust::P().units[cls];
+    return WUST_OK;
+}
+
 int wust_profile_reset(void) {
     try {
         wust::prof_flush();
@@ -141,6 +156,7 @@ int wust_profile_reset(void) {
     for (int c = 0; c < wust::PC_COUNT; ++c) {
         wust::P().count[c] = 0;
         wust::P().total_ms[c] = 0;
+        wust::P().units[c] = 0;
     }
     return WUST_OK;
 }

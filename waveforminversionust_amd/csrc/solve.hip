@@ -387,7 +387,8 @@ int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
 static void solve_engine_unfused(const Operator &op, const cf *src, cf *out, cf *scratch, int Rpad, bool adjoint,
                                  hipStream_t s);
 
-void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad, bool adjoint, hipStream_t s) {
+void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad, bool adjoint, hipStream_t s,
+                  int lead, int trail) {
     static const bool unfused = [] {
         const char *e = std::getenv("WUST_SWEEP_UNFUSED");
         return e && e[0] == '1';
@@ -435,19 +436,25 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
         const int tpw = 4 / FUSED_KS[variant];
         fp.units = fp.ntasks * strips * (Rpad / 16) / tpw;
         fp.per_xcd = (fp.units + 7) / 8;
+        prof_units(PC_SWEEP_GEMM, fp.ntasks);
         if (adjoint) launch_fused_variant<A_CONJT>(variant, fp, s, true);
         else launch_fused_variant<A_COL>(variant, fp, s, true);
     };
     const int nl = p, nr = m - 1 - p;
     const int nsteps = nl > nr ? nl : nr;
+    // a chain's iterates before the first nonzero column of src are exactly zero: its
+    // elimination starts there (l0 / r0 steps in), and those columns' back substitution has no
+    // y_j term; each chain computes at least its last step, which the meeting block reads
+    const int l0 = std::max(0, std::min(lead, nl - 1)), r0 = std::max(0, std::min(trail, nr - 1));
+    const int nfwd = std::max(nl - l0, nr - r0);
     // elimination from both ends: y_j = X_j B_j, and B_{next} from y_j in the epilogue
-    for (int st = 0; st < nsteps; ++st) {
+    for (int k = 0; k < nfwd; ++k) {
         std::vector<FusedTask> tasks;
-        if (st < nl) {
+        if (int st = l0 + k; st < nl) {
             const int j = st;
             FusedTask tk{};
             tk.X = op.page(j);
-            tk.B = (st == 0) ? S(j) : OP(0, st & 1);
+            tk.B = (st == l0) ? S(j) : OP(0, st & 1);
             tk.out = Y(j);
             if (st + 1 < nl) {                       // next left block j + 1 (not the meeting block)
                 tk.Bn = OP(0, (st + 1) & 1);
@@ -457,11 +464,11 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
             }
             tasks.push_back(tk);
         }
-        if (st < nr) {
+        if (int st = r0 + k; st < nr) {
             const int j = m - 1 - st;
             FusedTask tk{};
             tk.X = op.page(j);
-            tk.B = (st == 0) ? S(j) : OP(1, st & 1);
+            tk.B = (st == r0) ? S(j) : OP(1, st & 1);
             tk.out = Y(j);
             if (st + 1 < nr) {                       // next right block j - 1
                 tk.Bn = OP(1, (st + 1) & 1);
@@ -496,6 +503,7 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
             if (!gemm) return;
             sp.units = sp.ntasks * (npad / (16 * SWEEP_RB)) * (Rpad / 16);
             sp.per_xcd = (sp.units + 7) / 8;
+            prof_units(PC_SWEEP_GEMM, sp.ntasks);
             if (adjoint)
                 launch_k(PC_SWEEP_GEMM, sweep_gemm_kernel<SWEEP_RB, A_CONJT>, dim3(8 * sp.per_xcd), dim3(256), 0, s,
                          sp);
@@ -544,7 +552,7 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
             FusedTask tk{};
             tk.X = op.page(j);
             tk.B = OP(0, (st + 1) & 1);
-            tk.base = Y(j);
+            tk.base = (j >= l0) ? Y(j) : nullptr;
             tk.out = O(j);
             if (st + 1 < nl) {                       // next left block j - 1
                 tk.Bn = OP(0, st & 1);
@@ -558,7 +566,7 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
             FusedTask tk{};
             tk.X = op.page(j);
             tk.B = OP(1, (st + 1) & 1);
-            tk.base = Y(j);
+            tk.base = (j <= m - 1 - r0) ? Y(j) : nullptr;
             tk.out = O(j);
             if (st + 1 < nr) {                       // next right block j + 1
                 tk.Bn = OP(1, st & 1);
@@ -599,6 +607,7 @@ static void solve_engine_unfused(const Operator &op, const cf *src, cf *out, cf 
         launch_k(PC_SWEEP_OPERAND, sweep_operand_kernel, dim3((unsigned)((nops + 255) / 256)), dim3(256), 0, s, sp);
         sp.units = sp.ntasks * (npad / (16 * SWEEP_RB)) * (Rpad / 16);
         sp.per_xcd = (sp.units + 7) / 8;
+        prof_units(PC_SWEEP_GEMM, sp.ntasks);
         if (adjoint)
             launch_k(PC_SWEEP_GEMM, sweep_gemm_kernel<SWEEP_RB, A_CONJT>, dim3(8 * sp.per_xcd), dim3(256), 0, s, sp);
         else
