@@ -178,7 +178,7 @@ struct wust_helmholtz {
     PlainLib plain;
     int ws_Rpad = 0;
     cd *b64 = nullptr, *t64 = nullptr, *u64 = nullptr, *lam64 = nullptr;
-    cf *s32 = nullptr, *d32 = nullptr, *b32 = nullptr;
+    cf *s32 = nullptr, *d32 = nullptr, *d32b = nullptr, *b32 = nullptr;
     cf *p_src = nullptr, *p_u = nullptr, *p_tmp = nullptr;
     double *norms = nullptr;    // [2][Rpad]
     double *loss_r = nullptr;   // [Rpad]
@@ -229,6 +229,7 @@ struct wust_helmholtz {
         dpool(lam64, cnt);
         if (s32) dpool(s32, cnt);
         if (d32) dpool(d32, cnt);
+        if (d32b) dpool(d32b, cnt);
         if (b32) dpool(b32, solve_work_elems(op, ws_Rpad));
     }
     void release_retained() {
@@ -245,6 +246,7 @@ struct wust_helmholtz {
         dfree(lam64);
         dfree(s32);
         dfree(d32);
+        dfree(d32b);
         dfree(b32);
         dfree(p_src);
         dfree(p_u);
@@ -270,6 +272,7 @@ struct wust_helmholtz {
         } else {
             s32 = dalloc<cf>(cnt);
             d32 = dalloc<cf>(cnt);
+            if (op.mode == WUST_MODE_REFINED) d32b = dalloc<cf>(cnt);
             b32 = dalloc<cf>(solve_work_elems(op, Rpad));
         }
         norms = dalloc<double>(2 * (int64_t)Rpad);
@@ -319,11 +322,12 @@ struct wust_helmholtz {
         // one pass: U_new = U_old + D, R = b - H U_new, per-RHS |R|^2 (and |b|^2
         // on the first pass); returns the max over RHS of |R| / |b|
         bool first_pass = true;
-        auto residual_pass = [&](const cd *u_old, const cf *d, cd *u_new, cf *res32) {
+        auto residual_pass = [&](const cd *u_old, const cf *d0, const cf *d, cd *u_new, cf *res32) {
             WUST_HIP_CHECK(hipMemsetAsync(first_pass ? nb2 : nr2, 0, sizeof(double) * Rpad * (first_pass ? 2 : 1), s));
             ResidualArgs a{};
             a.w64 = op.w64;
             a.u_old = u_old;
+            a.d0 = d0;
             a.d32 = d;
             a.b64 = b;
             a.u_new = u_new;
@@ -351,7 +355,7 @@ struct wust_helmholtz {
             solve_plain(op, plain, p_src, p_u, p_tmp, R, adjoint, s);
             plain32_to_solve64(p_u, out, op.m, op.npad, R, Rpad, s);
             if (want_residual) {
-                st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr);
+                st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr, nullptr);
             }
             return st;
         }
@@ -360,28 +364,35 @@ struct wust_helmholtz {
         solve_engine(op, s32, d32, b32, Rpad, adjoint, s, lead, trail);
         if (op.mode == WUST_MODE_C64) {
             promote(d32, out, cnt, s);
-            if (want_residual) st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr);
+            if (want_residual) st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr, nullptr);
             return st;
         }
-        // refinement: U_new = U_old + D; R = b - H U_new; D = solve(R).  The
-        // iterates alternate between two buffers, ordered so that the usual
-        // single correction ends in `out` without a copy.
+        // refinement: U = D_1 + D_2 + ...; R = b - H U; D_{k+1} = solve(R).  The first pass keeps
+        // U = D_1 in single precision (d32, not stored in FP64) and the first correction goes to a
+        // second buffer (d32b), so the second pass reads two c64 fields instead of an FP64 one;
+        // later passes accumulate in FP64, alternating between two buffers ordered so that the
+        // usual single correction ends in `out` without a copy.
         cd *bufs[2] = {t64 == out ? u64 : t64, out};
-        const cd *u_old = nullptr;
-        int cur = 0;
-        int it = 0;
-        for (;; ++it) {
-            double rel = residual_pass(u_old, d32, bufs[cur], s32);
-            if (it == 0) st.initial_residual = rel;
-            st.final_residual = rel;
-            if (!(rel > tol) || it >= max_iter) break;
-            solve_engine(op, s32, d32, b32, Rpad, adjoint, s);
-            u_old = bufs[cur];
-            cur ^= 1;
+        double rel = residual_pass(nullptr, nullptr, d32, nullptr, s32);
+        st.initial_residual = st.final_residual = rel;
+        int it = 0, cur = 1;
+        if (!(rel > tol) || max_iter <= 0) {
+            promote(d32, out, cnt, s);
+        } else {
+            solve_engine(op, s32, d32b, b32, Rpad, adjoint, s);
+            it = 1;
+            rel = st.final_residual = residual_pass(nullptr, d32, d32b, bufs[cur], s32);
+            while (rel > tol && it < max_iter) {
+                solve_engine(op, s32, d32, b32, Rpad, adjoint, s);
+                ++it;
+                const cd *u_old = bufs[cur];
+                cur ^= 1;
+                rel = st.final_residual = residual_pass(u_old, nullptr, d32, bufs[cur], s32);
+            }
+            if (bufs[cur] != out)
+                WUST_HIP_CHECK(hipMemcpyAsync(out, bufs[cur], sizeof(cd) * cnt, hipMemcpyDeviceToDevice, s));
         }
         st.iterations = it;
-        if (bufs[cur] != out)
-            WUST_HIP_CHECK(hipMemcpyAsync(out, bufs[cur], sizeof(cd) * cnt, hipMemcpyDeviceToDevice, s));
         if (!std::isfinite(st.final_residual))
             throw Error(WUST_NUMERICAL_FAILURE, "non-finite residual in the refined solve");
         if (st.final_residual > tol && max_iter > 0)

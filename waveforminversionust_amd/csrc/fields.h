@@ -9,6 +9,7 @@ __host__ __device__ __forceinline__ cd tod_any(cf v) { return tod(v); }
 struct ResidualArgs {
     const cd *w64;        // [9][m][n]
     const cd *u_old;      // solve layout c128 or null
+    const cf *d0;         // c64 iterate (the first correction) in place of u_old, or null
     const cf *d32;        // correction c64 or null
     const cd *b64;        // right-hand side c128 or null (= 0)
     cd *u_new;            // U_old + D (may be null)

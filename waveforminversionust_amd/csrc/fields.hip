@@ -96,8 +96,9 @@ __constant__ int c_opp[9] = {DC, DS, DN, DE, DSE, DNE, DW, DSW, DNW};
 // unconditionally, so all of them are in flight at once.  A clamped node lies
 // inside the thread's own stencil, and its zero coefficient adds an exact
 // zero.  Fused u += d; per-RHS |r|^2 and |b|^2 reduced per workgroup in LDS
-// before the atomics.
-template <bool HAS_U, bool HAS_D>
+// before the atomics.  The iterate is U_old (c128), D0 (c64: the first correction, kept in
+// single precision so that the first pass stores no FP64 iterate) or zero.
+template <bool HAS_U, bool HAS_D0, bool HAS_D>
 __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
     constexpr int DY[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
     constexpr int DX[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
@@ -137,6 +138,7 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
                 const int64_t on = ((int64_t)xx * npad + yy) * Rpad + r;
                 u[dd] = mkd(0, 0);
                 if (HAS_U) u[dd] = a.u_old[on];
+                if (HAS_D0) u[dd] = tod(a.d0[on]);
                 if (HAS_D) u[dd] = u[dd] + tod(a.d32[on]);
             }
             const cd b = a.b64 ? a.b64[o] : mkd(0, 0);
@@ -386,10 +388,13 @@ void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int 
 }
 void launch_residual(const ResidualArgs &a, hipStream_t s) {
     dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);
-    if (a.u_old && a.d32) launch_k(PC_RESIDUAL, residual_kernel<true, true>, grid, dim3(256), 0, s, a);
-    else if (a.u_old) launch_k(PC_RESIDUAL, residual_kernel<true, false>, grid, dim3(256), 0, s, a);
-    else if (a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, true>, grid, dim3(256), 0, s, a);
-    else launch_k(PC_RESIDUAL, residual_kernel<false, false>, grid, dim3(256), 0, s, a);
+    require(!(a.u_old && a.d0), "residual: U_old and D0 are exclusive");
+    if (a.u_old && a.d32) launch_k(PC_RESIDUAL, residual_kernel<true, false, true>, grid, dim3(256), 0, s, a);
+    else if (a.u_old) launch_k(PC_RESIDUAL, residual_kernel<true, false, false>, grid, dim3(256), 0, s, a);
+    else if (a.d0 && a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, true, true>, grid, dim3(256), 0, s, a);
+    else if (a.d0) launch_k(PC_RESIDUAL, residual_kernel<false, true, false>, grid, dim3(256), 0, s, a);
+    else if (a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, false, true>, grid, dim3(256), 0, s, a);
+    else launch_k(PC_RESIDUAL, residual_kernel<false, false, false>, grid, dim3(256), 0, s, a);
 }
 void promote(const cf *in, cd *out, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(promote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
