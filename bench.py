@@ -405,6 +405,7 @@ def main():
     if largest != dominant:
         log(f"note: largest kernel class of the profiled step is {largest}, roofline stays on {dominant}")
     dom_idx = N.PROFILE_CLASSES.index(dominant)
+    lib.wust_profile_enable(0)
     lib.wust_profile_reset()
 
     # ---- timed region: K refined gradient evaluations; the dominant kernel's
