@@ -179,6 +179,8 @@ struct wust_helmholtz {
     int ws_Rpad = 0;
     cd *b64 = nullptr, *t64 = nullptr, *u64 = nullptr, *lam64 = nullptr;
     cf *s32 = nullptr, *d32 = nullptr, *d32b = nullptr, *b32 = nullptr;
+    uint8_t *bmask = nullptr;   // per 64 values of the current right-hand side: any nonzero
+    bool bmask_valid = false;
     cf *p_src = nullptr, *p_u = nullptr, *p_tmp = nullptr;
     double *norms = nullptr;    // [2][Rpad]
     double *loss_r = nullptr;   // [Rpad]
@@ -230,6 +232,7 @@ struct wust_helmholtz {
         if (s32) dpool(s32, cnt);
         if (d32) dpool(d32, cnt);
         if (d32b) dpool(d32b, cnt);
+        if (bmask) dpool(bmask, cnt / 64);
         if (b32) dpool(b32, solve_work_elems(op, ws_Rpad));
     }
     void release_retained() {
@@ -247,6 +250,7 @@ struct wust_helmholtz {
         dfree(s32);
         dfree(d32);
         dfree(d32b);
+        dfree(bmask);
         dfree(b32);
         dfree(p_src);
         dfree(p_u);
@@ -273,6 +277,7 @@ struct wust_helmholtz {
             s32 = dalloc<cf>(cnt);
             d32 = dalloc<cf>(cnt);
             if (op.mode == WUST_MODE_REFINED) d32b = dalloc<cf>(cnt);
+            bmask = dalloc<uint8_t>(cnt / 64);
             b32 = dalloc<cf>(solve_work_elems(op, Rpad));
         }
         norms = dalloc<double>(2 * (int64_t)Rpad);
@@ -299,13 +304,15 @@ struct wust_helmholtz {
     // sweeps skip their forward elimination)
     void demote_rhs(const cd *b, int64_t cnt, int Rpad, hipStream_t s, int &lead, int &trail) {
         lead = trail = 0;
+        bmask_valid = false;
         const char *e = std::getenv("WUST_SKIP_ZERO_COLUMNS");
-        if (!op.colflags || (e && e[0] == '0')) {
+        if (!op.colflags || !bmask || (e && e[0] == '0')) {
             demote(b, s32, cnt, s);
             return;
         }
         WUST_HIP_CHECK(hipMemsetAsync(op.colflags, 0, sizeof(int) * op.m, s));
-        demote_flag(b, s32, cnt, (int64_t)op.npad * Rpad, op.colflags, s);
+        demote_flag(b, s32, cnt, (int64_t)op.npad * Rpad, op.colflags, bmask, s);
+        bmask_valid = true;
         std::vector<int> f((size_t)op.m);
         WUST_HIP_CHECK(hipMemcpyAsync(f.data(), op.colflags, sizeof(int) * op.m, hipMemcpyDeviceToHost, s));
         WUST_HIP_CHECK(hipStreamSynchronize(s));
@@ -319,6 +326,7 @@ struct wust_helmholtz {
         wust_solve_stats st{};
         const int64_t cnt = field_elems(Rpad);
         double *nb2 = norms, *nr2 = norms + Rpad;
+        bmask_valid = false;   // set by demote_rhs for this b
         // one pass: U_new = U_old + D, R = b - H U_new, per-RHS |R|^2 (and |b|^2
         // on the first pass); returns the max over RHS of |R| / |b|
         bool first_pass = true;
@@ -330,6 +338,7 @@ struct wust_helmholtz {
             a.d0 = d0;
             a.d32 = d;
             a.b64 = b;
+            a.bmask = bmask_valid ? bmask : nullptr;
             a.u_new = u_new;
             a.res32 = res32;
             a.norm2 = nr2;

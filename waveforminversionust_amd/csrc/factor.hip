@@ -85,8 +85,7 @@ __global__ void __launch_bounds__(256) schur_kernel(SchurParams p) {
     const int rows[3] = {max(r - 1, 0), r, min(r + 1, n - 1)};
     const int64_t cols[3] = {(int64_t)ld * c, (int64_t)ld * max(c - 1, 0), (int64_t)ld * min(c + 1, n - 1)};
     const cf zero = mk(0.f, 0.f);
-    for (int t = 0; t < tk.nterms; ++t) {
-        const SchurTerm &st = tk.t[t];
+    auto term = [&](const SchurTerm &st) {
         const cf qm = (c >= 1) ? st.Qs[c - 1] : zero;
         const cf q0 = st.Qc[c];
         const cf qp = (c + 1 < n) ? st.Qn[c + 1] : zero;
@@ -105,7 +104,9 @@ __global__ void __launch_bounds__(256) schur_kernel(SchurParams p) {
             cfma(acc, pr[a], xq);
         }
         val = val - acc;
-    }
+    };
+    if (tk.nterms >= 1) term(tk.t[0]);
+    if (tk.nterms >= 2) term(tk.t[1]);
     tk.T[r + (int64_t)ld * c] = val;
 }
 

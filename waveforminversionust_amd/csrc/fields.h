@@ -12,6 +12,7 @@ struct ResidualArgs {
     const cf *d0;         // c64 iterate (the first correction) in place of u_old, or null
     const cf *d32;        // correction c64 or null
     const cd *b64;        // right-hand side c128 or null (= 0)
+    const uint8_t *bmask; // null, or per 64 consecutive values of b64: 0 when all of them are zero
     cd *u_new;            // U_old + D (may be null)
     cf *res32;            // B - H U_new as c64 (may be null)
     cd *res64;            // same in c128 (may be null)
@@ -66,8 +67,9 @@ void launch_residual(const ResidualArgs &a, hipStream_t s);
 void promote(const cf *in, cd *out, int64_t count, hipStream_t s);
 void demote(const cd *in, cf *out, int64_t count, hipStream_t s);
 // demote, and flags[i / per] = 1 for every block of `per` consecutive values that holds a nonzero
-// (flags pre-zeroed; per a multiple of 64)
-void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, hipStream_t s);
+// (flags pre-zeroed; per a multiple of 64); mask[i / 64] = whether values 64 (i / 64) .. + 63 hold
+// a nonzero (count a multiple of 64)
+void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, uint8_t *mask, hipStream_t s);
 void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s);
 void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s);
 void launch_gradient(const GradientArgs &a, hipStream_t s);

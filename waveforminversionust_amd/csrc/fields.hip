@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
                 if (HAS_D0) u[dd] = tod(a.d0[on]);
                 if (HAS_D) u[dd] = u[dd] + tod(a.d32[on]);
             }
-            const cd b = a.b64 ? a.b64[o] : mkd(0, 0);
+            const cd b = (a.b64 && !(a.bmask && !a.bmask[o >> 6])) ? a.b64[o] : mkd(0, 0);
             cd hu = mkd(0, 0);
 #pragma unroll
             for (int dd = 0; dd < 9; ++dd) hu = hu + wt[dd][ly] * u[dd];
@@ -180,7 +180,7 @@ __global__ void demote_kernel(const cd *__restrict__ in, cf *__restrict__ out, i
 }
 // A wave's 64 values lie in one block of `per` (a multiple of 64), so one lane stores the flag.
 __global__ void demote_flag_kernel(const cd *__restrict__ in, cf *__restrict__ out, int64_t count, int64_t per,
-                                   int *__restrict__ flags) {
+                                   int *__restrict__ flags, uint8_t *__restrict__ mask) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     bool nz = false;
     if (i < count) {
@@ -188,7 +188,11 @@ __global__ void demote_flag_kernel(const cd *__restrict__ in, cf *__restrict__ o
         out[i] = tof(v);
         nz = (v.re != 0.0) || (v.im != 0.0);
     }
-    if (__any(nz) && (threadIdx.x & 63) == 0) flags[i / per] = 1;
+    const bool any = __any(nz);
+    if ((threadIdx.x & 63) == 0 && i < count) {
+        if (any) flags[i / per] = 1;
+        mask[i >> 6] = any ? 1 : 0;
+    }
 }
 
 // unit impulses: B[xi][yi][r] = 1 at the source node of RHS r (B pre-zeroed)
@@ -400,9 +404,10 @@ void promote(const cf *in, cd *out, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(promote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
     WUST_LAUNCH_CHECK();
 }
-void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, hipStream_t s) {
+void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, uint8_t *mask, hipStream_t s) {
+    require(count % 64 == 0 && per % 64 == 0, "demote_flag: sizes must be multiples of 64");
     hipLaunchKernelGGL(demote_flag_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count,
-                       per, flags);
+                       per, flags, mask);
     WUST_LAUNCH_CHECK();
 }
 void demote(const cd *in, cf *out, int64_t count, hipStream_t s) {
