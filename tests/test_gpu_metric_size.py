@@ -1,0 +1,152 @@
+"""Size-independent checks at the metric size (needs an MI355X): 560 x 560 nodes (512^2 interior +
+24-cell PML), dx 0.25 mm, 0.8 MHz, 256 transmitters / receivers on the 56 mm ring, smooth random
+model (the refined grad-eval workload).  The FP64 oracle's sparse LU is too slow to run per test at
+this size, so each check is one the mathematics fixes:
+
+* assembly: the nine coefficient arrays against the oracle's vectorised FP64 assembly, <= 1e-13;
+* forward refined solve: the FP64 residual ||H u - b|| / ||b|| of all 256 point sources, computed
+  by an independent matrix-free apply of the reference's stencil
+  (Runtime/solver/HelmholtzSolver.m:125-128; the apply is first pinned against the oracle's sparse
+  H on a small grid), within the refinement tolerance;
+* adjoint refined solve: duality <b_f, H^-H b_a> = <H^-1 b_f, b_a> for the 256 x 256 pairs of
+  transmitter impulses and receiver-supported right-hand sides, <= 5e-8 relative (both solves
+  are accurate to ~2e-9 in the field);
+* gradient: central differences along two smooth directions with Richardson extrapolation,
+  <= 1e-5 relative (loss noise of the refined solves over the step).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import helmholtz_fp64 as H  # noqa: E402
+from waveforminversionust_amd import helmholtz as G  # noqa: E402
+from waveforminversionust_amd import workloads as W  # noqa: E402
+
+OPTS = dict(backend="gpu", stencil_bounds=[1400, 1700])
+
+
+def apply_H(co, U):
+    """H U for U [Ny, Nx, R] (complex128 tensors): interior rows by the nine-point stencil with the
+    reference's coefficient lookups, border rows the identity."""
+    Ny, Nx = U.shape[0], U.shape[1]
+    n, m = Ny - 2, Nx - 2
+    c = {k: v[..., None] for k, v in co.items()}
+    HU = U.clone()
+    HU[1:n + 1, 1:m + 1] = (c["Dd"][1:n + 1, 1:m + 1] * U[1:n + 1, 1:m + 1]
+                            + c["Dl"][0:n, 1:m + 1] * U[0:n, 1:m + 1]
+                            + c["Du"][1:n + 1, 1:m + 1] * U[2:n + 2, 1:m + 1]
+                            + c["Ld"][1:n + 1, 0:m] * U[1:n + 1, 0:m]
+                            + c["Ll"][0:n, 0:m] * U[0:n, 0:m]
+                            + c["Lu"][1:n + 1, 0:m] * U[2:n + 2, 0:m]
+                            + c["Ud"][1:n + 1, 1:m + 1] * U[1:n + 1, 2:m + 2]
+                            + c["Ul"][0:n, 1:m + 1] * U[0:n, 2:m + 2]
+                            + c["Uu"][1:n + 1, 1:m + 1] * U[2:n + 2, 2:m + 2])
+    return HU
+
+
+def test_stencil_apply_matches_oracle_sparse_H():
+    x = np.arange(11) * 1e-3
+    y = np.arange(9) * 1.1e-3
+    rng = np.random.default_rng(4)
+    c = 1500 + 30 * rng.standard_normal((9, 11))
+    ref = H.HelmholtzFP64(x, y, c, np.zeros_like(c), 2e5, -1, 10, 2.5e-3, dict(stencil_bounds=[1400, 1700]))
+    U = rng.standard_normal((9, 11, 3)) + 1j * rng.standard_normal((9, 11, 3))
+    want = (ref.H @ U.reshape(-1, 3, order="F")).reshape(9, 11, 3, order="F")
+    co = {k: torch.as_tensor(v, device="cuda") for k, v in ref.coef.items()}
+    got = apply_H(co, torch.as_tensor(U, device="cuda")).cpu().numpy()
+    assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+
+
+@pytest.fixture(scope="module")
+def metric():
+    case = W.metric_case()
+    f = case.frequencies_hz[0]
+    s = G.HelmholtzSolver(case.x_m, case.y_m, case.c_true, None, f, -1, case.cfg["pml_strength"], case.cfg["pml_m"],
+                          dict(OPTS, mode="refined"))
+    yield case, s
+    s.close()
+
+
+def _impulses(case, index, amplitudes=None):
+    b = torch.zeros((case.nx * case.ny, index.size), dtype=torch.complex128, device="cuda")
+    cols = torch.arange(index.size, device="cuda")
+    b[torch.as_tensor(index - 1, device="cuda"), cols] = (torch.ones(index.size, dtype=torch.complex128,
+                                                                     device="cuda") if amplitudes is None
+                                                          else amplitudes)
+    return b.reshape(case.nx, case.ny, -1).permute(1, 0, 2)      # [Ny, Nx, R], linear index y + Ny x
+
+
+def test_metric_coefficients_match_oracle(metric):
+    case, s = metric
+    # the oracle's assembly steps without its sparse LU (which takes seconds at this size)
+    f, lo, hi = case.frequencies_hz[0], case.cfg["stencil_bounds"][0], case.cfg["stencil_bounds"][1]
+    t = H.pml_terms(case.x_m, case.y_m, case.c_true, np.zeros_like(case.c_true), f, -1, case.cfg["pml_strength"],
+                    case.cfg["pml_m"])
+    b, d, e = H.stencil_opt_params(lo, hi, f, t.h, t.g)
+    want = H.nine_point_coefficients(t.A, t.B, t.C, t.k, b, d, e, t.h, t.g, case.nx, case.ny)
+    co = s.coefficients()
+    for k, v in want.items():
+        got = co[k].cpu().numpy()
+        assert got.shape == v.shape
+        assert np.max(np.abs(got - v)) <= 1e-13 * np.max(np.abs(v)), k
+
+
+def test_metric_refined_forward_residual(metric):
+    case, s = metric
+    b = _impulses(case, case.tx_index)
+    u, _, st = s.solve(b, False, return_stats=True)
+    r = apply_H(s.coefficients(), u) - b
+    per_rhs = torch.linalg.vector_norm(r, dim=(0, 1)) / torch.linalg.vector_norm(b, dim=(0, 1))
+    worst = float(per_rhs.max())
+    assert st["iterations"] >= 1 and worst <= s.tol * 1.01, (worst, st)
+    assert abs(worst - st["final_residual"]) <= 1e-12, (worst, st)
+
+
+def test_metric_adjoint_duality(metric):
+    case, s = metric
+    rng = np.random.default_rng(17)
+    amp = torch.as_tensor(rng.standard_normal(case.n_rx) + 1j * rng.standard_normal(case.n_rx), device="cuda")
+    b_f = _impulses(case, case.tx_index)
+    u, _ = s.solve(b_f, False)
+    # receiver-supported right-hand sides: column j holds amp at receiver j and amp/2 at receiver j+1
+    idx = torch.as_tensor(case.rx_index - 1, device="cuda")
+    b_a = torch.zeros((case.nx * case.ny, case.n_rx), dtype=torch.complex128, device="cuda")
+    cols = torch.arange(case.n_rx, device="cuda")
+    b_a[idx, cols] = amp
+    b_a[idx.roll(-1), cols] += amp / 2
+    b_a = b_a.reshape(case.nx, case.ny, -1).permute(1, 0, 2)
+    lam, _ = s.solve(b_a, True)
+    lam_flat = lam.permute(1, 0, 2).reshape(-1, case.n_rx)
+    u_flat = u.permute(1, 0, 2).reshape(-1, case.n_tx)
+    s1 = lam_flat[torch.as_tensor(case.tx_index - 1, device="cuda"), :]            # <b_f_i, lam_j>
+    s2 = u_flat.conj().T @ b_a.permute(1, 0, 2).reshape(-1, case.n_rx)             # <u_i, b_a_j>
+    rel = float(torch.linalg.norm(s1 - s2) / torch.linalg.norm(s1))
+    assert rel <= 5e-8, rel
+
+
+def test_metric_gradient_central_differences(metric):
+    case, s = metric
+    u, _ = s.solve(_impulses(case, case.tx_index), False)
+    Y = u.permute(1, 0, 2).reshape(-1, case.n_tx)[torch.as_tensor(case.rx_index - 1, device="cuda"), :].T
+    obs = dict(x_m=case.x_m, y_m=case.y_m, shape_yx=(case.ny, case.nx), n_tx=case.n_tx, tx_index=case.tx_index,
+               rx_index=case.rx_index, frequencies_hz=list(case.frequencies_hz), Y=Y.contiguous()[:, :, None],
+               mask=torch.ones((case.n_tx, case.n_rx, 1), dtype=torch.bool, device="cuda"))
+    cfg = dict(case.cfg, tol=1e-12)
+    s0 = torch.full((case.ny, case.nx), 1 / 1500, dtype=torch.float64, device="cuda")
+    loss0, g = G.wust_oracle(1 / s0, obs, 0, cfg)
+    rng = np.random.default_rng(23)
+    for _ in range(2):
+        d = W.gaussian_filter2(rng.standard_normal((case.ny, case.nx)), 12.0)
+        d = torch.as_tensor(d / np.max(np.abs(d)), device="cuda") * s0
+        an = float(torch.sum(g * d))
+
+        def fd(h):
+            lp = G.wust_oracle(1 / (s0 + h * d), obs, 0, cfg)[0]
+            lm = G.wust_oracle(1 / (s0 - h * d), obs, 0, cfg)[0]
+            return (lp - lm) / (2 * h)
+
+        h = 1e-3
+        rich = (4 * fd(h / 2) - fd(h)) / 3
+        assert abs(rich - an) <= 1e-5 * abs(an), (rich, an)
