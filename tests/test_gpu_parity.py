@@ -6,7 +6,7 @@ Tolerances (stated per test):
   the oracle's own sparse H, and forward error against SuperLU within
   cond-scaled bounds;
 * refined loss and gradient vs the FP64 oracle: <= 1e-8 relative (the
-  north-star target) on the 160x160 / 64-TX case;
+  gradient accuracy target) on the 160x160 / 64-TX case;
 * unrefined complex-FP32 modes ('c64' engine, 'plain' vendor-library
   comparator): <= 1e-3 relative, the complex-single level of the reference's
   GPU path (Runtime/tests/test_runtime.m:46-52 accepts 0.03 on its FD check).
