@@ -254,6 +254,10 @@ int wust_profile_reset(void);
  * GEMM class, block chain steps (one n x n by n x R complex product each), so that a caller can
  * price the class's time against the work actually executed. */
 int wust_profile_units(int kernel_class, int64_t *units);
+/* Time one launch in `stride` of each profiled class (default 1: every launch); resets the
+ * per-class launch counters.  An event-timed launch's completion signal idles the device for
+ * several microseconds before the next launch, so a run that is itself being timed samples. */
+int wust_profile_stride(uint32_t stride);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,7 @@ EXPORTS = {
     "wust_profile_read": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "wust_profile_reset": (C.c_int, []),
     "wust_profile_units": (C.c_int, [C.c_int, C.POINTER(C.c_int64)]),
+    "wust_profile_stride": (C.c_int, [C.c_uint32]),
     "wust_roi_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                     C.c_void_p]),

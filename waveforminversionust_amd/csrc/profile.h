@@ -26,15 +26,21 @@ void prof_record(int cls, bool begin, hipStream_t s);
 void prof_flush();
 hipEvent_t prof_event();
 void prof_push(int cls, hipEvent_t a, hipEvent_t b);
-// count n work units (the class's own unit, e.g. chain steps of a sweep launch) when cls is profiled
+// announce the work units (the class's own unit, e.g. chain steps of a sweep launch) of the
+// class's next launch; they are counted if that launch is timed
 void prof_units(int cls, int64_t n);
+// whether this launch of cls is timed: the class is profiled and the launch is the first of each
+// `stride` (wust_profile_stride) since the last reset
+bool prof_take(int cls);
 
-// Launch a kernel; when its class is profiled, the dispatch packet itself
-// timestamps start and end (hipExtLaunchKernelGGL events), so the timing adds
-// no packets to the stream.
+// Launch a kernel; when its class is profiled (and this launch is sampled), the
+// dispatch packet itself timestamps start and end (hipExtLaunchKernelGGL
+// events), so the timing adds no packets to the stream.  The completion signal
+// of such a launch leaves the device idle for several microseconds before the
+// next one, which is why a timed run samples one launch in many.
 template <typename K, typename... Args>
 void launch_k(int cls, K kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s, Args... args) {
-    if (prof_on(cls)) {
+    if (prof_take(cls)) {
         hipEvent_t a = prof_event(), b = prof_event();
         hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, a, b, 0, args...);
         prof_push(cls, a, b);

@@ -27,6 +27,9 @@ struct Profiler {
     double total_ms[PC_COUNT] = {};
     int64_t count[PC_COUNT] = {};
     int64_t units[PC_COUNT] = {};   // work units of the profiled launches (per class; see prof_units)
+    int64_t announced[PC_COUNT] = {};  // units announced for a class's next launch
+    int64_t calls[PC_COUNT] = {};    // launches of a profiled class seen (for sampling)
+    uint32_t stride = 1;             // time one launch in `stride` of each profiled class
     Profiler() : open(PC_COUNT, nullptr) {}
     // resolve one pending pair (blocking when `wait`); false if it has not completed
     bool resolve_front(bool wait) {
@@ -103,7 +106,17 @@ void prof_units(int cls, int64_t n) {
     if (!prof_on(cls)) return;
     Profiler &p = P();
     std::lock_guard<std::mutex> lk(p.mu);
-    p.units[cls] += n;
+    p.announced[cls] = n;
+}
+
+bool prof_take(int cls) {
+    if (!prof_on(cls)) return false;
+    Profiler &p = P();
+    std::lock_guard<std::mutex> lk(p.mu);
+    const bool take = (p.calls[cls]++ % p.stride) == 0;
+    if (take) p.units[cls] += p.announced[cls];
+    p.announced[cls] = 0;
+    return take;
 }
 
 void prof_flush() {
@@ -140,6 +153,14 @@ int wust_profile_read(int cls, int64_t *count, double *total_ms) {
     return WUST_OK;
 }
 
+int wust_profile_stride(uint32_t stride) {
+    if (stride < 1) return WUST_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(wust::P().mu);
+    wust::P().stride = stride;
+    for (int c = 0; c < wust::PC_COUNT; ++c) wust::P().calls[c] = 0;
+    return WUST_OK;
+}
+
 int wust_profile_units(int cls, int64_t *units) {
     if (cls < 0 || cls >= wust::PC_COUNT || !units) return WUST_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(wust::P().mu);
@@ -157,6 +178,8 @@ int wust_profile_reset(void) {
         wust::P().count[c] = 0;
         wust::P().total_ms[c] = 0;
         wust::P().units[c] = 0;
+        wust::P().announced[c] = 0;
+        wust::P().calls[c] = 0;
     }
     return WUST_OK;
 }
