@@ -182,7 +182,7 @@ struct FusedParams {
 // so a workgroup of four waves covers TPW = 4 / KS adjacent RHS tiles of one strip; DEPTH
 // k-blocks of operands are in flight per wave.  The variant is picked per shape at run time.
 template <int ALAY, int DEPTH, int KS>
-__global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
+__device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
     constexpr int TPW = 4 / KS;
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
     const int L = xcd * p.per_xcd + local;
@@ -283,6 +283,19 @@ __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
     }
 }
 
+// 1280 workgroups of four waves (40 strips x 16 RHS tiles x 2 chains at 558 rows and 256 RHS)
+// need five resident waves per SIMD to run in one round; with two k-blocks in flight the step fits
+// that register budget without spilling (deeper prefetch does not, and keeps its own budget).
+template <int ALAY, int KS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+sweep_fused_kernel_d2(FusedParams p) {
+    sweep_fused_body<ALAY, 2, KS>(p);
+}
+template <int ALAY, int DEPTH, int KS>
+__global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
+    sweep_fused_body<ALAY, DEPTH, KS>(p);
+}
+
 // Zeroes rows r0 .. npad-1 of the [npad][Rpad] blocks of two arrays (blockIdx.y selects the
 // array): the padding rows that the fused steps' strips do not cover, which the steps read as
 // operand rows (against zero columns of the pages) and as tile halos (against zero couplings),
@@ -316,8 +329,8 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
     switch (v) {
         case 1: go(sweep_fused_kernel<ALAY, 3, 4>); break;
         case 2: go(sweep_fused_kernel<ALAY, 4, 2>); break;
-        case 3: go(sweep_fused_kernel<ALAY, 2, 2>); break;
-        default: go(sweep_fused_kernel<ALAY, 2, 4>); break;
+        case 3: go(sweep_fused_kernel_d2<ALAY, 2>); break;
+        default: go(sweep_fused_kernel_d2<ALAY, 4>); break;
     }
 }
 
