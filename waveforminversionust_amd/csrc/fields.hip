@@ -107,41 +107,59 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
     const int yi = blockIdx.y * 16 + ly;
     const int xi = blockIdx.z;
     const int n = a.n, m = a.m, npad = a.npad, Rpad = a.Rpad;
-    __shared__ cd wt[9][16];
-    if (threadIdx.x < 144) {
-        const int dd = threadIdx.x >> 4, ty = blockIdx.y * 16 + (threadIdx.x & 15);
-        const int yy = ty + c_dy[dd], xx = xi + c_dx[dd];
-        const int64_t plane = (int64_t)m * n;
-        cd w = mkd(0, 0);
-        if (ty < n && yy >= 0 && yy < n && xx >= 0 && xx < m) {
-            if (!a.adjoint) w = a.w64[dd * plane + (int64_t)xi * n + ty];
-            else w = conjd(a.w64[c_opp[dd] * plane + (int64_t)xx * n + yy]);
+    const int64_t o = ((int64_t)xi * npad + yi) * Rpad + r;
+    // Straight-line loads, in this order: the stencil coefficient (threads < 144 stage one), the
+    // right-hand-side mask byte and the nine neighbours.  Every thread loads from in-range
+    // (clamped) addresses and selects afterwards, so that waiting for the coefficient before
+    // staging it in LDS (loads complete in order) leaves the neighbour loads in flight across the
+    // barrier, which waits for LDS only.  Rows beyond the interior read row n - 1 and discard it.
+    const int t = threadIdx.x < 144 ? threadIdx.x : 0;
+    const int cdir = t >> 4, ty = blockIdx.y * 16 + (t & 15);
+    const int iy = cdir % 3, ix = cdir / 3;                        // direction d = 3 ix + iy
+    const int cy = iy == 1 ? -1 : (iy == 2 ? 1 : 0), cx = ix == 1 ? -1 : (ix == 2 ? 1 : 0);
+    const int yy = ty + cy, xx = xi + cx;
+    const bool wvalid = ty < n && yy >= 0 && yy < n && xx >= 0 && xx < m;
+    const int64_t plane = (int64_t)m * n;
+    const int opp = 3 * (ix == 1 ? 2 : (ix == 2 ? 1 : 0)) + (iy == 1 ? 2 : (iy == 2 ? 1 : 0));
+    const cd wl = !a.adjoint ? a.w64[cdir * plane + (int64_t)xi * n + min(ty, n - 1)]
+                             : a.w64[opp * plane + (int64_t)min(max(xx, 0), m - 1) * n + min(max(yy, 0), n - 1)];
+    const int yl = min(yi, n - 1);
+    const int64_t ol = ((int64_t)xi * npad + yl) * Rpad + r;
+    const uint8_t bm = a.bmask ? a.bmask[ol >> 6] : uint8_t(1);
+    cd ur[9];
+    cf d0r[9], dr[9];
+    {
+        const int ys[3] = {yl, max(yl - 1, 0), min(yl + 1, n - 1)};
+        const int xs[3] = {xi, max(xi - 1, 0), min(xi + 1, m - 1)};
+#pragma unroll
+        for (int dd = 0; dd < 9; ++dd) {
+            const int yq = ys[DY[dd] < 0 ? 1 : (DY[dd] > 0 ? 2 : 0)];
+            const int xq = xs[DX[dd] < 0 ? 1 : (DX[dd] > 0 ? 2 : 0)];
+            const int64_t on = ((int64_t)xq * npad + yq) * Rpad + r;
+            if (HAS_U) ur[dd] = a.u_old[on];
+            if (HAS_D0) d0r[dd] = a.d0[on];
+            if (HAS_D) dr[dd] = a.d32[on];
         }
-        wt[dd][threadIdx.x & 15] = w;
     }
-    __syncthreads();
+    __shared__ cd wt[9][16];
+    if (threadIdx.x < 144) wt[cdir][threadIdx.x & 15] = wvalid ? (a.adjoint ? conjd(wl) : wl) : mkd(0, 0);
+    lds_barrier();
     double contrib = 0, bcontrib = 0;
     if (yi < npad && r < Rpad) {
-        const int64_t o = ((int64_t)xi * npad + yi) * Rpad + r;
         if (yi >= n) {
             if (a.u_new) a.u_new[o] = mkd(0, 0);
             if (a.res32) a.res32[o] = mk(0.f, 0.f);
             if (a.res64) a.res64[o] = mkd(0, 0);
         } else {
-            const int ys[3] = {yi, max(yi - 1, 0), min(yi + 1, n - 1)};
-            const int xs[3] = {xi, max(xi - 1, 0), min(xi + 1, m - 1)};
+            const cd b = (a.b64 && bm) ? a.b64[o] : mkd(0, 0);
             cd u[9];
 #pragma unroll
             for (int dd = 0; dd < 9; ++dd) {
-                const int yy = ys[DY[dd] < 0 ? 1 : (DY[dd] > 0 ? 2 : 0)];
-                const int xx = xs[DX[dd] < 0 ? 1 : (DX[dd] > 0 ? 2 : 0)];
-                const int64_t on = ((int64_t)xx * npad + yy) * Rpad + r;
                 u[dd] = mkd(0, 0);
-                if (HAS_U) u[dd] = a.u_old[on];
-                if (HAS_D0) u[dd] = tod(a.d0[on]);
-                if (HAS_D) u[dd] = u[dd] + tod(a.d32[on]);
+                if (HAS_U) u[dd] = ur[dd];
+                if (HAS_D0) u[dd] = tod(d0r[dd]);
+                if (HAS_D) u[dd] = u[dd] + tod(dr[dd]);
             }
-            const cd b = (a.b64 && !(a.bmask && !a.bmask[o >> 6])) ? a.b64[o] : mkd(0, 0);
             cd hu = mkd(0, 0);
 #pragma unroll
             for (int dd = 0; dd < 9; ++dd) hu = hu + wt[dd][ly] * u[dd];
