@@ -357,39 +357,48 @@ int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
         auto it = g_fused_choice.find(key);
         if (it != g_fused_choice.end()) return it->second;
     }
+    // Each eligible variant runs BURST back-to-back launches (as the sweep runs them) per round,
+    // rounds interleaving the variants; a variant's time is its best round (the least disturbed).
+    constexpr int ROUNDS = 3, BURST = 8;
     hipEvent_t e0, e1;
     WUST_HIP_CHECK(hipEventCreate(&e0));
     WUST_HIP_CHECK(hipEventCreate(&e1));
-    int best = 0;
-    float best_ms = 1e30f;
+    FusedParams q[N_FUSED_VARIANTS];
+    bool ok[N_FUSED_VARIANTS];
+    float per_launch[N_FUSED_VARIANTS];
+    auto run = [&](int v) {
+        if (adjoint) launch_fused_variant<A_CONJT>(v, q[v], s, false);
+        else launch_fused_variant<A_COL>(v, q[v], s, false);
+    };
     for (int v = 0; v < N_FUSED_VARIANTS; ++v) {
         const int tpw = 4 / FUSED_KS[v];
-        if ((fp.Rpad / 16) % tpw) continue;
-        FusedParams q = fp;
-        q.units = q.ntasks * q.strips * (q.Rpad / 16) / tpw;
-        q.per_xcd = (q.units + 7) / 8;
-        float total = 0.f;
-        for (int rep = 0; rep < 4; ++rep) {
+        ok[v] = (fp.Rpad / 16) % tpw == 0;
+        per_launch[v] = 1e30f;
+        q[v] = fp;
+        q[v].units = q[v].ntasks * q[v].strips * (q[v].Rpad / 16) / tpw;
+        q[v].per_xcd = (q[v].units + 7) / 8;
+        if (ok[v]) run(v);    // warms the code object and the caches
+    }
+    for (int round = 0; round < ROUNDS; ++round)
+        for (int v = 0; v < N_FUSED_VARIANTS; ++v) {
+            if (!ok[v]) continue;
             WUST_HIP_CHECK(hipEventRecord(e0, s));
-            if (adjoint) launch_fused_variant<A_CONJT>(v, q, s, false);
-            else launch_fused_variant<A_COL>(v, q, s, false);
+            for (int b = 0; b < BURST; ++b) run(v);
             WUST_HIP_CHECK(hipEventRecord(e1, s));
             WUST_HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
             WUST_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            if (rep > 0) total += ms;    // the first launch warms the caches and the code object
+            per_launch[v] = std::min(per_launch[v], ms / BURST);
         }
-        if (total < best_ms) {
-            best_ms = total;
-            best = v;
-        }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    int best = 0;
+    for (int v = 0; v < N_FUSED_VARIANTS; ++v)
+        if (ok[v] && (!ok[best] || per_launch[v] < per_launch[best])) best = v;
     if (const char *e = std::getenv("WUST_AUTOTUNE_LOG"); e && e[0] == '1')
         std::fprintf(stderr, "sweep step kernel: npad %d Rpad %d %s -> depth %d, K split %d (%.2f us)\n", fp.npad,
                      fp.Rpad, adjoint ? "adjoint" : "forward", best == 1 ? 3 : best == 2 ? 4 : 2, FUSED_KS[best],
-                     1e3f * best_ms / 3.f);
+                     1e3f * per_launch[best]);
     std::lock_guard<std::mutex> lk(g_fused_mu);
     g_fused_choice[key] = best;
     return best;
