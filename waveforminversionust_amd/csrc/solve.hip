@@ -285,7 +285,9 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
 
 // 1280 workgroups of four waves (40 strips x 16 RHS tiles x 2 chains at 558 rows and 256 RHS)
 // need five resident waves per SIMD to run in one round; with two k-blocks in flight the step fits
-// that register budget without spilling (deeper prefetch does not, and keeps its own budget).
+// that register budget without spilling (deeper prefetch does not).  Launches that take several
+// rounds anyway (larger blocks or more RHS) run faster with the unconstrained build, so both are
+// autotuner variants.
 template <int ALAY, int KS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 sweep_fused_kernel_d2(FusedParams p) {
@@ -311,10 +313,12 @@ __global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r
 }
 
 // Kernel variants of the fused step and their per-shape choice.  Variant v: (DEPTH, KS) =
-// (2, 4), (3, 4), (4, 2), (2, 2); a variant with TPW tiles per workgroup needs Rpad / 16 to be
-// a multiple of TPW.
-constexpr int N_FUSED_VARIANTS = 4;
-constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2};
+// (2, 4) five waves, (3, 4), (4, 2), (2, 2) five waves, (2, 4), (2, 2); a variant with TPW tiles
+// per workgroup needs Rpad / 16 to be a multiple of TPW.
+constexpr int N_FUSED_VARIANTS = 6;
+constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2, 4, 2};
+constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {2, 3, 4, 2, 2, 2};
+constexpr bool FUSED_FIVE[N_FUSED_VARIANTS] = {true, false, false, true, false, false};
 
 template <int ALAY>
 void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool profiled) {
@@ -330,6 +334,8 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
         case 1: go(sweep_fused_kernel<ALAY, 3, 4>); break;
         case 2: go(sweep_fused_kernel<ALAY, 4, 2>); break;
         case 3: go(sweep_fused_kernel_d2<ALAY, 2>); break;
+        case 4: go(sweep_fused_kernel<ALAY, 2, 4>); break;
+        case 5: go(sweep_fused_kernel<ALAY, 2, 2>); break;
         default: go(sweep_fused_kernel_d2<ALAY, 4>); break;
     }
 }
@@ -396,9 +402,9 @@ int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
     for (int v = 0; v < N_FUSED_VARIANTS; ++v)
         if (ok[v] && (!ok[best] || per_launch[v] < per_launch[best])) best = v;
     if (const char *e = std::getenv("WUST_AUTOTUNE_LOG"); e && e[0] == '1')
-        std::fprintf(stderr, "sweep step kernel: npad %d Rpad %d %s -> depth %d, K split %d (%.2f us)\n", fp.npad,
-                     fp.Rpad, adjoint ? "adjoint" : "forward", best == 1 ? 3 : best == 2 ? 4 : 2, FUSED_KS[best],
-                     1e3f * per_launch[best]);
+        std::fprintf(stderr, "sweep step kernel: npad %d Rpad %d %s -> depth %d, K split %d%s (%.2f us)\n", fp.npad,
+                     fp.Rpad, adjoint ? "adjoint" : "forward", FUSED_DEPTH[best], FUSED_KS[best],
+                     FUSED_FIVE[best] ? ", five waves" : "", 1e3f * per_launch[best]);
     std::lock_guard<std::mutex> lk(g_fused_mu);
     g_fused_choice[key] = best;
     return best;
