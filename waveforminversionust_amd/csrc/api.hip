@@ -327,8 +327,9 @@ struct wust_helmholtz {
         const int64_t cnt = field_elems(Rpad);
         double *nb2 = norms, *nr2 = norms + Rpad;
         bmask_valid = false;   // set by demote_rhs for this b
-        // one pass: U_new = U_old + D, R = b - H U_new, per-RHS |R|^2 (and |b|^2
-        // on the first pass); returns the max over RHS of |R| / |b|
+        // one pass: U = U_old + D (FP64 U_old) or D0 + D (two c64 corrections), stored in U_new when
+        // given; R = b - H U, per-RHS |R|^2 (and |b|^2 on the first pass); returns the max over RHS
+        // of |R| / |b|
         bool first_pass = true;
         auto residual_pass = [&](const cd *u_old, const cf *d0, const cf *d, cd *u_new, cf *res32) {
             WUST_HIP_CHECK(hipMemsetAsync(first_pass ? nb2 : nr2, 0, sizeof(double) * Rpad * (first_pass ? 2 : 1), s));
