@@ -125,3 +125,37 @@ def test_zero_column_skip_is_exact(monkeypatch, mode, adjoint):
     interior[1:-1, 1:-1] = True
     res = (Hm @ u - bf).reshape(ny, nx, 9, order="F")[interior]
     assert np.linalg.norm(res) <= (1e-12 if mode == "refined" else 1e-4) * np.linalg.norm(bf)
+
+
+@pytest.mark.parametrize("where", ["zero", "first", "second", "before_meet", "meet", "after_meet", "last"])
+def test_zero_column_skip_edges(monkeypatch, where):
+    """The skip at its edges: an all-zero right-hand side, and right-hand sides confined to one
+    block column at either end of the chains, next to the meeting block or at it (the chains keep
+    at least their last step).  Bit-identical to the solve that runs every step, and solved."""
+    ny, nx = 20, 19                                       # m = 17 interior columns, meeting block p = 8
+    x, y, c = _grid(ny, nx)
+    m, p = nx - 2, (nx - 2) // 2
+    col = dict(zero=None, first=0, second=1, before_meet=p - 1, meet=p, after_meet=p + 1, last=m - 1)[where]
+    rng = np.random.default_rng(11)
+    b = np.zeros((ny, nx, 3), np.complex128)
+    if col is not None:
+        b[1:-1, col + 1] = rng.standard_normal((ny - 2, 3)) + 1j * rng.standard_normal((ny - 2, 3))
+    f, a0, L = 2.2e5, 10.0, 2e-3
+    ref = H.HelmholtzFP64(x, y, c, np.zeros_like(c), f, -1, a0, L, dict(stencil_bounds=[1400, 1700]))
+    s = G.HelmholtzSolver(x, y, c, None, f, -1, a0, L, dict(OPTS, mode="refined", tol=1e-13))
+    for adjoint in (False, True):
+        monkeypatch.delenv("WUST_SKIP_ZERO_COLUMNS", raising=False)
+        got = s.solve(b, adjoint)[0].clone()
+        monkeypatch.setenv("WUST_SKIP_ZERO_COLUMNS", "0")
+        full = s.solve(b, adjoint)[0].clone()
+        assert torch.equal(got, full), (where, adjoint)
+        u = got.cpu().numpy()
+        if col is None:
+            assert not np.any(u)
+            continue
+        Hm = ref.H.conj().T if adjoint else ref.H
+        interior = np.zeros((ny, nx), bool)
+        interior[1:-1, 1:-1] = True
+        bf = b.reshape(-1, 3, order="F")
+        res = (Hm @ u.reshape(-1, 3, order="F") - bf).reshape(ny, nx, 3, order="F")[interior]
+        assert np.linalg.norm(res) <= 1e-12 * np.linalg.norm(bf), (where, adjoint)
