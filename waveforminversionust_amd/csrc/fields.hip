@@ -89,88 +89,131 @@ __constant__ int c_dy[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
 __constant__ int c_dx[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
 __constant__ int c_opp[9] = {DC, DS, DN, DE, DSE, DNE, DW, DSW, DNW};
 
-// One thread per (RHS r, row y, column x), a workgroup per 16 RHS x 16 rows of
-// one column.  The workgroup's 9 x 16 coupling coefficients (the same for all
-// RHS) are staged once in LDS, zero for couplings that leave the interior;
-// the nine neighbour reads then go to clamped in-range nodes and are issued
-// unconditionally, so all of them are in flight at once.  A clamped node lies
-// inside the thread's own stencil, and its zero coefficient adds an exact
-// zero.  Fused u += d; per-RHS |r|^2 and |b|^2 reduced per workgroup in LDS
-// before the atomics.  The iterate is U_old (c128), D0 (c64: the first correction, kept in
-// single precision so that the first pass stores no FP64 iterate) or zero.
+// One thread per (RHS r, row y), a workgroup per 16 RHS x 16 rows that marches along a run of
+// RES_COLS columns.  Each thread keeps its stencil's 3 x 3 iterate values (columns x-1, x, x+1 at
+// rows y-1, y, y+1) in registers, so a column step loads one new column of three values per
+// iterate array, a step ahead of its use; neighbour addresses are clamped in range and the
+// coefficient of a coupling that leaves the interior is zero, so a clamped node (one inside the
+// thread's own stencil) adds an exact zero.  The workgroup's 9 x 16 coefficients of a column (the
+// same for all RHS) are staged in LDS, double-buffered with one barrier per column; its
+// right-hand-side mask bytes for the whole run are staged once.  Loads are issued in the order
+// they are consumed (right-hand side, iterates, coefficient), so waiting for one leaves the later
+// ones in flight.  Fused u += d; per-RHS |r|^2 and |b|^2 are summed over the run in registers and
+// reduced per workgroup in LDS before the atomics.  The iterate is U_old (c128), D0 (c64: the
+// first correction, kept in single precision so that the first pass stores no FP64 iterate) or
+// zero.
+constexpr int RES_COLS = 16;
+
 template <bool HAS_U, bool HAS_D0, bool HAS_D>
 __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
-    constexpr int DY[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
-    constexpr int DX[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
     const int lr = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int r = blockIdx.x * 16 + lr;
-    const int yi = blockIdx.y * 16 + ly;
-    const int xi = blockIdx.z;
+    const int y0 = blockIdx.y * 16, yi = y0 + ly;
     const int n = a.n, m = a.m, npad = a.npad, Rpad = a.Rpad;
-    const int64_t o = ((int64_t)xi * npad + yi) * Rpad + r;
-    // Straight-line loads, in this order: the stencil coefficient (threads < 144 stage one), the
-    // right-hand-side mask byte and the nine neighbours.  Every thread loads from in-range
-    // (clamped) addresses and selects afterwards, so that waiting for the coefficient before
-    // staging it in LDS (loads complete in order) leaves the neighbour loads in flight across the
-    // barrier, which waits for LDS only.  Rows beyond the interior read row n - 1 and discard it.
+    const int xa = blockIdx.z * RES_COLS, xb = min(xa + RES_COLS, m);
+    const int64_t colstride = (int64_t)npad * Rpad;
+    const int yl = min(yi, n - 1);
+    const int64_t rowoff[3] = {(int64_t)max(yl - 1, 0) * Rpad + r, (int64_t)yl * Rpad + r,
+                               (int64_t)min(yl + 1, n - 1) * Rpad + r};   // rows y-1, y, y+1 (clamped)
+    const bool live = yi < n;
+
+    // the coefficient this thread stages (threads < 144): direction cdir = 3 ix + iy of tile row t % 16
     const int t = threadIdx.x < 144 ? threadIdx.x : 0;
-    const int cdir = t >> 4, ty = blockIdx.y * 16 + (t & 15);
-    const int iy = cdir % 3, ix = cdir / 3;                        // direction d = 3 ix + iy
+    const int cdir = t >> 4, ty = y0 + (t & 15);
+    const int iy = cdir % 3, ix = cdir / 3;
     const int cy = iy == 1 ? -1 : (iy == 2 ? 1 : 0), cx = ix == 1 ? -1 : (ix == 2 ? 1 : 0);
-    const int yy = ty + cy, xx = xi + cx;
-    const bool wvalid = ty < n && yy >= 0 && yy < n && xx >= 0 && xx < m;
+    const int yy = ty + cy;
+    const bool yvalid = ty < n && yy >= 0 && yy < n;
     const int64_t plane = (int64_t)m * n;
     const int opp = 3 * (ix == 1 ? 2 : (ix == 2 ? 1 : 0)) + (iy == 1 ? 2 : (iy == 2 ? 1 : 0));
-    const cd wl = !a.adjoint ? a.w64[cdir * plane + (int64_t)xi * n + min(ty, n - 1)]
-                             : a.w64[opp * plane + (int64_t)min(max(xx, 0), m - 1) * n + min(max(yy, 0), n - 1)];
-    const int yl = min(yi, n - 1);
-    const int64_t ol = ((int64_t)xi * npad + yl) * Rpad + r;
-    const uint8_t bm = a.bmask ? a.bmask[ol >> 6] : uint8_t(1);
-    cd ur[9];
-    cf d0r[9], dr[9];
-    {
-        const int ys[3] = {yl, max(yl - 1, 0), min(yl + 1, n - 1)};
-        const int xs[3] = {xi, max(xi - 1, 0), min(xi + 1, m - 1)};
+    const cd *wsrc = !a.adjoint ? a.w64 + cdir * plane + min(ty, n - 1)
+                                : a.w64 + opp * plane + min(max(yy, 0), n - 1);
+    auto coef = [&](int x) -> cd {        // column x's coefficient (x in range; neighbour clamped)
+        const int xc = a.adjoint ? min(max(x + cx, 0), m - 1) : x;
+        return wsrc[(int64_t)xc * n];
+    };
+
+    auto load_col = [&](int x, cd (&u)[3], cf (&p)[3], cf (&q)[3]) {
+        const int64_t base = (int64_t)min(max(x, 0), m - 1) * colstride;
 #pragma unroll
-        for (int dd = 0; dd < 9; ++dd) {
-            const int yq = ys[DY[dd] < 0 ? 1 : (DY[dd] > 0 ? 2 : 0)];
-            const int xq = xs[DX[dd] < 0 ? 1 : (DX[dd] > 0 ? 2 : 0)];
-            const int64_t on = ((int64_t)xq * npad + yq) * Rpad + r;
-            if (HAS_U) ur[dd] = a.u_old[on];
-            if (HAS_D0) d0r[dd] = a.d0[on];
-            if (HAS_D) dr[dd] = a.d32[on];
+        for (int k = 0; k < 3; ++k) {
+            if (HAS_U) u[k] = a.u_old[base + rowoff[k]];
+            if (HAS_D0) p[k] = a.d0[base + rowoff[k]];
+            if (HAS_D) q[k] = a.d32[base + rowoff[k]];
         }
+    };
+
+    __shared__ uint8_t bms[RES_COLS][16];
+    __shared__ cd wt[2][9][16];
+    {   // mask byte of column xa + t / 16, tile row t % 16 (the 16 RHS share one byte)
+        const int bx = xa + (threadIdx.x >> 4), brow = min(y0 + (threadIdx.x & 15), n - 1);
+        uint8_t v = 1;
+        if (a.bmask && bx < xb) v = a.bmask[(((int64_t)bx * npad + brow) * Rpad + blockIdx.x * 16) >> 6];
+        bms[threadIdx.x >> 4][threadIdx.x & 15] = v;
     }
-    __shared__ cd wt[9][16];
-    if (threadIdx.x < 144) wt[cdir][threadIdx.x & 15] = wvalid ? (a.adjoint ? conjd(wl) : wl) : mkd(0, 0);
     lds_barrier();
+
+    const int64_t own = (int64_t)yi * Rpad + r;   // this thread's node in a column (unclamped row)
+    cd b = (live && a.b64 && bms[0][ly]) ? a.b64[(int64_t)xa * colstride + own] : mkd(0, 0);
+    cd wu[3][3];
+    cf wp[3][3], wq[3][3];
+    load_col(xa - 1, wu[0], wp[0], wq[0]);
+    load_col(xa, wu[1], wp[1], wq[1]);
+    load_col(xa + 1, wu[2], wp[2], wq[2]);
+    cd wl = coef(xa);
+
     double contrib = 0, bcontrib = 0;
-    if (yi < npad && r < Rpad) {
-        if (yi >= n) {
+#pragma unroll 1
+    for (int x = xa; x < xb; ++x) {
+        const int64_t o = (int64_t)x * colstride + own;
+        // the next column's loads, in the order the next step consumes them
+        cd bn = mkd(0, 0);
+        if (x + 1 < xb && live && a.b64 && bms[x + 1 - xa][ly]) bn = a.b64[o + colstride];
+        cd nu[3];
+        cf np[3], nq[3];
+        load_col(x + 2, nu, np, nq);
+        const cd wl_n = coef(min(x + 1, m - 1));
+
+        if (threadIdx.x < 144) {
+            const bool valid = yvalid && x + cx >= 0 && x + cx < m;
+            wt[x & 1][cdir][threadIdx.x & 15] = valid ? (a.adjoint ? conjd(wl) : wl) : mkd(0, 0);
+        }
+        lds_barrier();
+        if (!live) {
             if (a.u_new) a.u_new[o] = mkd(0, 0);
             if (a.res32) a.res32[o] = mk(0.f, 0.f);
             if (a.res64) a.res64[o] = mkd(0, 0);
         } else {
-            const cd b = (a.b64 && bm) ? a.b64[o] : mkd(0, 0);
+            // direction dd = 3 ix + iy: column x + DX[dd], row y + DY[dd]
+            constexpr int DY[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
+            constexpr int DX[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
             cd u[9];
 #pragma unroll
             for (int dd = 0; dd < 9; ++dd) {
+                const int wc = DX[dd] + 1, wr = DY[dd] + 1;
                 u[dd] = mkd(0, 0);
-                if (HAS_U) u[dd] = ur[dd];
-                if (HAS_D0) u[dd] = tod(d0r[dd]);
-                if (HAS_D) u[dd] = u[dd] + tod(dr[dd]);
+                if (HAS_U) u[dd] = wu[wc][wr];
+                if (HAS_D0) u[dd] = tod(wp[wc][wr]);
+                if (HAS_D) u[dd] = u[dd] + tod(wq[wc][wr]);
             }
             cd hu = mkd(0, 0);
 #pragma unroll
-            for (int dd = 0; dd < 9; ++dd) hu = hu + wt[dd][ly] * u[dd];
-            const cd self = u[DC];
-            cd res = b - hu;
-            if (a.u_new) a.u_new[o] = self;
+            for (int dd = 0; dd < 9; ++dd) hu = hu + wt[x & 1][dd][ly] * u[dd];
+            const cd res = b - hu;
+            if (a.u_new) a.u_new[o] = u[DC];
             if (a.res32) a.res32[o] = tof(res);
             if (a.res64) a.res64[o] = res;
-            contrib = abs2(res);
-            bcontrib = abs2(b);
+            contrib += abs2(res);
+            bcontrib += abs2(b);
         }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            wu[0][k] = wu[1][k]; wu[1][k] = wu[2][k]; wu[2][k] = nu[k];
+            wp[0][k] = wp[1][k]; wp[1][k] = wp[2][k]; wp[2][k] = np[k];
+            wq[0][k] = wq[1][k]; wq[1][k] = wq[2][k]; wq[2][k] = nq[k];
+        }
+        b = bn;
+        wl = wl_n;
     }
     if (a.norm2 || a.bnorm2) {
         __shared__ double part[2][16][17];
@@ -183,7 +226,7 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
             double s = 0;
             for (int k = 0; k < 16; ++k) s += part[which][k][lane];
             const int rr = blockIdx.x * 16 + lane;
-            if (dst && rr < Rpad && s != 0) atomicAdd(dst + rr, s);
+            if (dst && s != 0) atomicAdd(dst + rr, s);
         }
     }
 }
@@ -409,8 +452,9 @@ void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int 
     WUST_LAUNCH_CHECK();
 }
 void launch_residual(const ResidualArgs &a, hipStream_t s) {
-    dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);
+    dim3 grid(a.Rpad / 16, a.npad / 16, (a.m + RES_COLS - 1) / RES_COLS);
     require(!(a.u_old && a.d0), "residual: U_old and D0 are exclusive");
+    require(a.Rpad % 16 == 0 && a.npad % 16 == 0 && a.n >= 1 && a.n <= a.npad, "residual: padded sizes");
     if (a.u_old && a.d32) launch_k(PC_RESIDUAL, residual_kernel<true, false, true>, grid, dim3(256), 0, s, a);
     else if (a.u_old) launch_k(PC_RESIDUAL, residual_kernel<true, false, false>, grid, dim3(256), 0, s, a);
     else if (a.d0 && a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, true, true>, grid, dim3(256), 0, s, a);
