@@ -90,18 +90,18 @@ __constant__ int c_dx[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
 __constant__ int c_opp[9] = {DC, DS, DN, DE, DSE, DNE, DW, DSW, DNW};
 
 // One thread per (RHS r, row y), a workgroup per 16 RHS x 16 rows that marches along a run of
-// RES_COLS columns.  Each thread keeps its stencil's 3 x 3 iterate values (columns x-1, x, x+1 at
-// rows y-1, y, y+1) in registers, so a column step loads one new column of three values per
-// iterate array, a step ahead of its use; neighbour addresses are clamped in range and the
-// coefficient of a coupling that leaves the interior is zero, so a clamped node (one inside the
-// thread's own stencil) adds an exact zero.  The workgroup's 9 x 16 coefficients of a column (the
-// same for all RHS) are staged in LDS, double-buffered with one barrier per column; its
-// right-hand-side mask bytes for the whole run are staged once.  Loads are issued in the order
-// they are consumed (right-hand side, iterates, coefficient), so waiting for one leaves the later
-// ones in flight.  Fused u += d; per-RHS |r|^2 and |b|^2 are summed over the run in registers and
-// reduced per workgroup in LDS before the atomics.  The iterate is U_old (c128), D0 (c64: the
-// first correction, kept in single precision so that the first pass stores no FP64 iterate) or
-// zero.
+// RES_COLS columns.  The iterate values of columns x-1, x, x+1 over the tile's rows and one halo
+// row on each side sit in an LDS ring of four column slots: a column step writes the column that
+// the previous step loaded (each thread its own row, 32 threads the halo rows), issues the loads
+// of the column after it, and after one barrier reads its nine neighbours from the ring, so every
+// iterate value is loaded from memory once per workgroup and the loads stay in flight for a whole
+// step.  Neighbour addresses are clamped in range and the coefficient of a coupling that leaves
+// the interior is zero, so a clamped node (one inside the thread's own stencil) adds an exact zero.
+// The 9 x 16 coefficients of a column (the same for all RHS) are staged the same way,
+// double-buffered; the right-hand-side mask bytes of the whole run are staged once.  Fused
+// u += d; per-RHS |r|^2 and |b|^2 are summed over the run in registers and reduced per workgroup
+// in LDS before the atomics.  The iterate is U_old (c128), D0 (c64: the first correction, kept in
+// single precision so that the first pass stores no FP64 iterate) or zero.
 constexpr int RES_COLS = 16;
 
 template <bool HAS_U, bool HAS_D0, bool HAS_D>
@@ -112,10 +112,13 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
     const int n = a.n, m = a.m, npad = a.npad, Rpad = a.Rpad;
     const int xa = blockIdx.z * RES_COLS, xb = min(xa + RES_COLS, m);
     const int64_t colstride = (int64_t)npad * Rpad;
-    const int yl = min(yi, n - 1);
-    const int64_t rowoff[3] = {(int64_t)max(yl - 1, 0) * Rpad + r, (int64_t)yl * Rpad + r,
-                               (int64_t)min(yl + 1, n - 1) * Rpad + r};   // rows y-1, y, y+1 (clamped)
     const bool live = yi < n;
+    // ring rows j = 0 .. 17 hold grid rows y0 - 1 + j, clamped to the interior; thread t holds its
+    // own row (j = t / 16 + 1) and threads t < 32 also the halo rows j = 0 (t < 16) and j = 17
+    const bool halo = threadIdx.x < 32;
+    const int hj = threadIdx.x < 16 ? 0 : 17;
+    const int64_t own_l = (int64_t)min(yi, n - 1) * Rpad + r;
+    const int64_t halo_l = (int64_t)(threadIdx.x < 16 ? max(y0 - 1, 0) : min(y0 + 16, n - 1)) * Rpad + r;
 
     // the coefficient this thread stages (threads < 144): direction cdir = 3 ix + iy of tile row t % 16
     const int t = threadIdx.x < 144 ? threadIdx.x : 0;
@@ -133,87 +136,96 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
         return wsrc[(int64_t)xc * n];
     };
 
-    auto load_col = [&](int x, cd (&u)[3], cf (&p)[3], cf (&q)[3]) {
+    __shared__ cd ru[HAS_U ? 4 : 1][18][16];
+    __shared__ cf rp[HAS_D0 ? 4 : 1][18][16];
+    __shared__ cf rq[HAS_D ? 4 : 1][18][16];
+    __shared__ uint8_t bms[RES_COLS][16];
+    __shared__ cd wt[2][9][16];
+    cd pu = mkd(0, 0), hu = mkd(0, 0);   // loaded column: own row, halo row
+    cf pp = mk(0.f, 0.f), hp = mk(0.f, 0.f), pq = mk(0.f, 0.f), hq = mk(0.f, 0.f);
+    auto load_col = [&](int x) {
         const int64_t base = (int64_t)min(max(x, 0), m - 1) * colstride;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (HAS_U) u[k] = a.u_old[base + rowoff[k]];
-            if (HAS_D0) p[k] = a.d0[base + rowoff[k]];
-            if (HAS_D) q[k] = a.d32[base + rowoff[k]];
+        if (HAS_U) pu = a.u_old[base + own_l];
+        if (HAS_D0) pp = a.d0[base + own_l];
+        if (HAS_D) pq = a.d32[base + own_l];
+        if (halo) {
+            if (HAS_U) hu = a.u_old[base + halo_l];
+            if (HAS_D0) hp = a.d0[base + halo_l];
+            if (HAS_D) hq = a.d32[base + halo_l];
+        }
+    };
+    auto store_col = [&](int x) {         // the loaded column into its ring slot (x may be -1)
+        const int s = x & 3;
+        if (HAS_U) ru[s][ly + 1][lr] = pu;
+        if (HAS_D0) rp[s][ly + 1][lr] = pp;
+        if (HAS_D) rq[s][ly + 1][lr] = pq;
+        if (halo) {
+            if (HAS_U) ru[s][hj][lr] = hu;
+            if (HAS_D0) rp[s][hj][lr] = hp;
+            if (HAS_D) rq[s][hj][lr] = hq;
         }
     };
 
-    __shared__ uint8_t bms[RES_COLS][16];
-    __shared__ cd wt[2][9][16];
     {   // mask byte of column xa + t / 16, tile row t % 16 (the 16 RHS share one byte)
         const int bx = xa + (threadIdx.x >> 4), brow = min(y0 + (threadIdx.x & 15), n - 1);
         uint8_t v = 1;
         if (a.bmask && bx < xb) v = a.bmask[(((int64_t)bx * npad + brow) * Rpad + blockIdx.x * 16) >> 6];
         bms[threadIdx.x >> 4][threadIdx.x & 15] = v;
     }
+    load_col(xa - 1);
+    store_col(xa - 1);
+    load_col(xa);
+    store_col(xa);
     lds_barrier();
-
     const int64_t own = (int64_t)yi * Rpad + r;   // this thread's node in a column (unclamped row)
-    cd b = (live && a.b64 && bms[0][ly]) ? a.b64[(int64_t)xa * colstride + own] : mkd(0, 0);
-    cd wu[3][3];
-    cf wp[3][3], wq[3][3];
-    load_col(xa - 1, wu[0], wp[0], wq[0]);
-    load_col(xa, wu[1], wp[1], wq[1]);
-    load_col(xa + 1, wu[2], wp[2], wq[2]);
+    load_col(xa + 1);
     cd wl = coef(xa);
+    cd bn = (live && a.b64 && bms[0][ly]) ? a.b64[(int64_t)xa * colstride + own] : mkd(0, 0);
 
     double contrib = 0, bcontrib = 0;
 #pragma unroll 1
     for (int x = xa; x < xb; ++x) {
         const int64_t o = (int64_t)x * colstride + own;
-        // the next column's loads, in the order the next step consumes them
-        cd bn = mkd(0, 0);
-        if (x + 1 < xb && live && a.b64 && bms[x + 1 - xa][ly]) bn = a.b64[o + colstride];
-        cd nu[3];
-        cf np[3], nq[3];
-        load_col(x + 2, nu, np, nq);
-        const cd wl_n = coef(min(x + 1, m - 1));
-
+        // stage what the previous step loaded (column x + 1, column x's coefficient and b) ...
+        store_col(x + 1);
         if (threadIdx.x < 144) {
             const bool valid = yvalid && x + cx >= 0 && x + cx < m;
             wt[x & 1][cdir][threadIdx.x & 15] = valid ? (a.adjoint ? conjd(wl) : wl) : mkd(0, 0);
         }
+        const cd b = bn;
+        // ... then issue the next step's loads, which stay in flight through this step
+        load_col(x + 2);
+        wl = coef(min(x + 1, m - 1));
+        bn = mkd(0, 0);
+        if (x + 1 < xb && live && a.b64 && bms[x + 1 - xa][ly]) bn = a.b64[o + colstride];
         lds_barrier();
         if (!live) {
             if (a.u_new) a.u_new[o] = mkd(0, 0);
             if (a.res32) a.res32[o] = mk(0.f, 0.f);
             if (a.res64) a.res64[o] = mkd(0, 0);
-        } else {
-            // direction dd = 3 ix + iy: column x + DX[dd], row y + DY[dd]
-            constexpr int DY[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
-            constexpr int DX[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
-            cd u[9];
-#pragma unroll
-            for (int dd = 0; dd < 9; ++dd) {
-                const int wc = DX[dd] + 1, wr = DY[dd] + 1;
-                u[dd] = mkd(0, 0);
-                if (HAS_U) u[dd] = wu[wc][wr];
-                if (HAS_D0) u[dd] = tod(wp[wc][wr]);
-                if (HAS_D) u[dd] = u[dd] + tod(wq[wc][wr]);
-            }
-            cd hu = mkd(0, 0);
-#pragma unroll
-            for (int dd = 0; dd < 9; ++dd) hu = hu + wt[x & 1][dd][ly] * u[dd];
-            const cd res = b - hu;
-            if (a.u_new) a.u_new[o] = u[DC];
-            if (a.res32) a.res32[o] = tof(res);
-            if (a.res64) a.res64[o] = res;
-            contrib += abs2(res);
-            bcontrib += abs2(b);
+            continue;
         }
+        // direction dd = 3 ix + iy: column x + DX[dd], row y + DY[dd]
+        constexpr int DY[9] = {0, -1, 1, 0, -1, 1, 0, -1, 1};
+        constexpr int DX[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
+        cd u[9];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            wu[0][k] = wu[1][k]; wu[1][k] = wu[2][k]; wu[2][k] = nu[k];
-            wp[0][k] = wp[1][k]; wp[1][k] = wp[2][k]; wp[2][k] = np[k];
-            wq[0][k] = wq[1][k]; wq[1][k] = wq[2][k]; wq[2][k] = nq[k];
+        for (int dd = 0; dd < 9; ++dd) {
+            const int s = (x + DX[dd]) & 3, j = ly + 1 + DY[dd];
+            u[dd] = mkd(0, 0);
+            if (HAS_U) u[dd] = ru[s][j][lr];
+            if (HAS_D0) u[dd] = tod(rp[s][j][lr]);
+            if (HAS_D) u[dd] = u[dd] + tod(rq[s][j][lr]);
         }
-        b = bn;
-        wl = wl_n;
+        cd huacc = mkd(0, 0);
+#pragma unroll
+        for (int dd = 0; dd < 9; ++dd) huacc = huacc + wt[x & 1][dd][ly] * u[dd];
+        const cd res = b - huacc;
+        if (a.u_new) a.u_new[o] = u[DC];
+        if (a.res32) a.res32[o] = tof(res);
+        if (a.res64) a.res64[o] = res;
+        contrib += abs2(res);
+        bcontrib += abs2(b);
     }
     if (a.norm2 || a.bnorm2) {
         __shared__ double part[2][16][17];
