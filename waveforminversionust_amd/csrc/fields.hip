@@ -102,7 +102,7 @@ __constant__ int c_opp[9] = {DC, DS, DN, DE, DSE, DNE, DW, DSW, DNW};
 // u += d; per-RHS |r|^2 and |b|^2 are summed over the run in registers and reduced per workgroup
 // in LDS before the atomics.  The iterate is U_old (c128), D0 (c64: the first correction, kept in
 // single precision so that the first pass stores no FP64 iterate) or zero.
-constexpr int RES_COLS = 16;
+constexpr int RES_COLS = 32;
 
 template <bool HAS_U, bool HAS_D0, bool HAS_D>
 __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
@@ -166,11 +166,12 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
         }
     };
 
-    {   // mask byte of column xa + t / 16, tile row t % 16 (the 16 RHS share one byte)
-        const int bx = xa + (threadIdx.x >> 4), brow = min(y0 + (threadIdx.x & 15), n - 1);
+    for (int e = threadIdx.x; e < RES_COLS * 16; e += 256) {
+        // mask byte of column xa + e / 16, tile row e % 16 (the 16 RHS share one byte)
+        const int bx = xa + (e >> 4), brow = min(y0 + (e & 15), n - 1);
         uint8_t v = 1;
         if (a.bmask && bx < xb) v = a.bmask[(((int64_t)bx * npad + brow) * Rpad + blockIdx.x * 16) >> 6];
-        bms[threadIdx.x >> 4][threadIdx.x & 15] = v;
+        bms[e >> 4][e & 15] = v;
     }
     load_col(xa - 1);
     store_col(xa - 1);
@@ -212,10 +213,12 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
 #pragma unroll
         for (int dd = 0; dd < 9; ++dd) {
             const int s = (x + DX[dd]) & 3, j = ly + 1 + DY[dd];
-            u[dd] = mkd(0, 0);
-            if (HAS_U) u[dd] = ru[s][j][lr];
-            if (HAS_D0) u[dd] = tod(rp[s][j][lr]);
-            if (HAS_D) u[dd] = u[dd] + tod(rq[s][j][lr]);
+            if (HAS_U || HAS_D0) {
+                u[dd] = HAS_U ? ru[s][j][lr] : tod(rp[s][j][lr]);
+                if (HAS_D) u[dd] = u[dd] + tod(rq[s][j][lr]);
+            } else {
+                u[dd] = HAS_D ? tod(rq[s][j][lr]) : mkd(0, 0);
+            }
         }
         cd huacc = mkd(0, 0);
 #pragma unroll
