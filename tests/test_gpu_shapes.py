@@ -159,3 +159,40 @@ def test_zero_column_skip_edges(monkeypatch, where):
         bf = b.reshape(-1, 3, order="F")
         res = (Hm @ u.reshape(-1, 3, order="F") - bf).reshape(ny, nx, 3, order="F")[interior]
         assert np.linalg.norm(res) <= 1e-12 * np.linalg.norm(bf), (where, adjoint)
+
+
+@pytest.mark.parametrize("ny,nx", [(66, 40), (52, 37)])
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
+    """Every kernel variant of the fused sweep step, forced in turn: the variants that split K over
+    four waves (16- or 32-row tiles, any prefetch depth) accumulate each element in the same order
+    and give bit-identical solutions; those that split K over two waves agree to single-precision
+    rounding.  Each solution solves the oracle's H to single-precision accuracy.  (52 rows: the
+    32-row tiles' last strip is clamped to the block end and the padding rows beyond the strips
+    are zero.)"""
+    x, y, c = _grid(ny, nx)
+    f, a0, L = 2.2e5, 10.0, 2e-3
+    R = 32
+    rng = np.random.default_rng(ny + nx)
+    b = np.zeros((ny, nx, R), np.complex128)
+    b[1:-1, 1:-1] = rng.standard_normal((ny - 2, nx - 2, R)) + 1j * rng.standard_normal((ny - 2, nx - 2, R))
+    ref = H.HelmholtzFP64(x, y, c, np.zeros_like(c), f, -1, a0, L, dict(stencil_bounds=[1400, 1700]))
+    s = G.HelmholtzSolver(x, y, c, None, f, -1, a0, L, dict(OPTS, mode="c64"))
+    Hm = ref.H.conj().T if adjoint else ref.H
+    bf = b.reshape(-1, R, order="F")
+    interior = np.zeros((ny, nx), bool)
+    interior[1:-1, 1:-1] = True
+    base = None
+    for v, ksplit4 in [(0, True), (1, True), (2, False), (3, False), (4, True), (5, False), (6, True), (7, True)]:
+        monkeypatch.setenv("WUST_SWEEP_VARIANT", str(v))
+        u = s.solve(b, adjoint)[0].clone()
+        if base is None:
+            base = u
+        elif ksplit4:
+            assert torch.equal(u, base), v
+        else:
+            rel = float(torch.linalg.norm(u - base) / torch.linalg.norm(base))
+            assert rel <= 1e-5, (v, rel)
+        res = (Hm @ u.cpu().numpy().reshape(-1, R, order="F") - bf).reshape(ny, nx, R, order="F")[interior]
+        assert np.linalg.norm(res) <= 1e-4 * np.linalg.norm(bf), v
+

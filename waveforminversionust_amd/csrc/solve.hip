@@ -157,10 +157,11 @@ constexpr int SWEEP_RB = 1;   // 16-row tiles: 2 * 36 * 16 = 1152 WGs at 560^2 x
 // computes rows i0 .. i0 + 15 with i0 = clamp(14 mb - 1, 0, npad - 16), so every owned row
 // has both neighbours in the tile (or outside the block, where they are zero).  The owned
 // rows of v are stored and B' is formed from the tile in LDS; the separate operand pass and
-// its launch disappear, at the cost of 16/14 of the MFMA work.  Back substitution reads
-// its base (y_j) from a different array than it writes (u_j), so halo rows are never read
-// after a neighbour rewrote them.
-constexpr int FUSED_OWN = 14;
+// its launch disappear, at the cost of 16/14 of the MFMA work (32-row tiles that own 30 rows
+// cost 32/30, with half the workgroups).  Back substitution reads its base (y_j) from a
+// different array than it writes (u_j), so halo rows are never read after a neighbour
+// rewrote them.
+constexpr int FUSED_OWN = 14;   // owned rows of a 16-row tile
 
 struct FusedTask {
     const cf *X;              // inverse page
@@ -174,7 +175,7 @@ struct FusedTask {
 };
 struct FusedParams {
     FusedTask task[2];
-    int ntasks, npad, Rpad;
+    int ntasks, n, npad, Rpad;
     int strips, units, per_xcd;
 };
 
@@ -283,6 +284,129 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
     }
 }
 
+// The same step on TR-row tiles (TR a multiple of 16; TR - 2 rows owned, a wave accumulating
+// TR / 16 row blocks so that each operand block it loads serves all of them).  Kept apart from
+// the 16-row body so that each compiles to its own register allocation.
+template <int ALAY, int DEPTH, int KS, int TR>
+__device__ __forceinline__ void sweep_fused_body_tall(const FusedParams &p) {
+    constexpr int TPW = 4 / KS, RB = TR / 16, OWN = TR - 2;
+    constexpr int EPT = RB * TPW;   // epilogue elements per thread
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int L = xcd * p.per_xcd + local;
+    if (L >= p.units) return;
+    const int npad = p.npad, Rpad = p.Rpad;
+    const int Ng = Rpad / 16 / TPW;
+    const int ng = L % Ng;
+    const int mb = (L / Ng) % p.strips;
+    const int task = L / (Ng * p.strips);
+    const FusedTask &tk = p.task[task];
+    const int o0 = OWN * mb;
+    const int o1 = min(o0 + OWN - 1, npad - 1);
+    const int i0 = min(max(o0 - 1, 0), npad - TR);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+    const int tw = wave / KS, ks = wave % KS;
+    const int j0 = (ng * TPW + tw) * 16;
+
+    // epilogue operands (base rows of this lane's tile, and for this thread's output elements the
+    // next source and the three coupling coefficients) are loaded before the GEMM so that their
+    // latency overlaps it instead of trailing it.  Element q of this thread is tile row
+    // (threadIdx.x / 16) + 16 (q % RB), column threadIdx.x % 16 of RHS tile q / RB.
+    cf pbase[RB][4];
+    if (tk.base && ks == 0) {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) pbase[rb][r] = tk.base[(int64_t)(i0 + 16 * rb + 4 * g + r) * Rpad + j0 + c];
+    }
+    const int cc = threadIdx.x & 15;
+    bool own[RB];
+    cf pn[RB], pc[RB], ps[RB], psrc[EPT];
+#pragma unroll
+    for (int rr = 0; rr < RB; ++rr) {
+        const int k = i0 + (threadIdx.x >> 4) + 16 * rr;
+        own[rr] = k >= o0 && k <= o1;
+        pn[rr] = pc[rr] = ps[rr] = mk(0.f, 0.f);
+        if (own[rr] && tk.Bn) {
+            if (!tk.transposed) {
+                pc[rr] = tk.cc[k];
+                if (k >= 1) pn[rr] = tk.cn[k];
+                if (k + 1 < npad) ps[rr] = tk.cs[k];
+            } else {
+                pc[rr] = conjf_(tk.cc[k]);
+                if (k >= 1) pn[rr] = conjf_(tk.cs[k - 1]);
+                if (k + 1 < npad) ps[rr] = conjf_(tk.cn[k + 1]);
+            }
+#pragma unroll
+            for (int tq = 0; tq < TPW; ++tq)
+                psrc[rr + RB * tq] = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + (ng * TPW + tq) * 16 + cc] : mk(0.f, 0.f);
+        }
+    }
+
+    CgemmWave<RB, 1, ALAY, B_ROW, false, DEPTH> acc;
+    acc.zero();
+    const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
+    acc.run(A, npad, tk.B + j0, Rpad, ks, npad / 16, KS, g, c);
+
+    __shared__ float4 red[4][RB][2][64];
+    __shared__ cf tile[TPW][TR][17];
+    float re[RB][4], im[RB][4];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc.result(rb, 0, re[rb], im[rb]);
+    if (KS > 1) {
+        if (ks > 0) {
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                red[wave][rb][0][lane] = make_float4(re[rb][0], re[rb][1], re[rb][2], re[rb][3]);
+                red[wave][rb][1][lane] = make_float4(im[rb][0], im[rb][1], im[rb][2], im[rb][3]);
+            }
+        }
+        __syncthreads();
+    }
+    if (ks == 0) {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+            for (int w = 1; w < KS; ++w) {
+                const float4 u = red[wave + w][rb][0][lane], v = red[wave + w][rb][1][lane];
+                re[rb][0] += u.x; re[rb][1] += u.y; re[rb][2] += u.z; re[rb][3] += u.w;
+                im[rb][0] += v.x; im[rb][1] += v.y; im[rb][2] += v.z; im[rb][3] += v.w;
+            }
+            // lane holds tile rows 16 rb + 4 g + r of column j0 + c
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                cf v = mk(re[rb][r], im[rb][r]);
+                if (tk.base) v = v + pbase[rb][r];
+                tile[tw][16 * rb + 4 * g + r][c] = v;
+            }
+        }
+    }
+    __syncthreads();
+    // epilogue: this thread's elements
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int rr = q % RB, tq = q / RB;
+        if (!own[rr]) continue;
+        const int t = (threadIdx.x >> 4) + 16 * rr, k = i0 + t;
+        const int64_t off = (int64_t)k * Rpad + (ng * TPW + tq) * 16 + cc;
+        const cf v = tile[tq][t][cc];
+        tk.out[off] = v;
+        if (!tk.Bn) continue;
+        const cf vn = (k >= 1) ? tile[tq][t - 1][cc] : mk(0.f, 0.f);        // row k - 1 (in the tile when k >= 1)
+        const cf vs = (k + 1 < npad) ? tile[tq][t + 1][cc] : mk(0.f, 0.f);  // row k + 1 (in the tile when < npad)
+        // the same products and order as operand_b: P v = c_c v_k + c_n v_{k-1} + c_s v_{k+1}
+        // (transposed: conj(cc_k) v_k + conj(cn_{k+1}) v_{k+1} + conj(cs_{k-1}) v_{k-1})
+        cf a = pc[rr] * v;
+        if (!tk.transposed) {
+            if (k >= 1) cfma(a, pn[rr], vn);
+            if (k + 1 < npad) cfma(a, ps[rr], vs);
+        } else {
+            if (k + 1 < npad) cfma(a, ps[rr], vs);
+            if (k >= 1) cfma(a, pn[rr], vn);
+        }
+        tk.Bn[off] = psrc[q] - a;
+    }
+}
+
 // 1280 workgroups of four waves (40 strips x 16 RHS tiles x 2 chains at 558 rows and 256 RHS)
 // need five resident waves per SIMD to run in one round; with two k-blocks in flight the step fits
 // that register budget without spilling (deeper prefetch does not).  Launches that take several
@@ -293,9 +417,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 sweep_fused_kernel_d2(FusedParams p) {
     sweep_fused_body<ALAY, 2, KS>(p);
 }
-template <int ALAY, int DEPTH, int KS>
+template <int ALAY, int DEPTH, int KS, int TR = 16>
 __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
-    sweep_fused_body<ALAY, DEPTH, KS>(p);
+    if constexpr (TR == 16) sweep_fused_body<ALAY, DEPTH, KS>(p);
+    else sweep_fused_body_tall<ALAY, DEPTH, KS, TR>(p);
 }
 
 // Zeroes rows r0 .. npad-1 of the [npad][Rpad] blocks of two arrays (blockIdx.y selects the
@@ -312,13 +437,19 @@ __global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r
     dst[blkno * npad * Rpad + (int64_t)r0 * Rpad + e] = mk(0.f, 0.f);
 }
 
-// Kernel variants of the fused step and their per-shape choice.  Variant v: (DEPTH, KS) =
-// (2, 4) five waves, (3, 4), (4, 2), (2, 2) five waves, (2, 4), (2, 2); a variant with TPW tiles
-// per workgroup needs Rpad / 16 to be a multiple of TPW.
-constexpr int N_FUSED_VARIANTS = 6;
-constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2, 4, 2};
-constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {2, 3, 4, 2, 2, 2};
-constexpr bool FUSED_FIVE[N_FUSED_VARIANTS] = {true, false, false, true, false, false};
+// Kernel variants of the fused step and their per-shape choice.  Variant v: (DEPTH, KS, tile
+// rows) = (2, 4, 16) five waves, (3, 4, 16), (4, 2, 16), (2, 2, 16) five waves, (2, 4, 16),
+// (2, 2, 16), (2, 4, 32), (3, 4, 32); a variant with TPW tiles per workgroup needs Rpad / 16 to
+// be a multiple of TPW.
+constexpr int N_FUSED_VARIANTS = 8;
+constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2, 4, 2, 4, 4};
+constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {2, 3, 4, 2, 2, 2, 2, 3};
+constexpr bool FUSED_FIVE[N_FUSED_VARIANTS] = {true, false, false, true, false, false, false, false};
+constexpr int FUSED_TR[N_FUSED_VARIANTS] = {16, 16, 16, 16, 16, 16, 32, 32};
+
+// strips of variant v's tiles over n rows (each owns FUSED_TR - 2 rows), and the rows they cover
+inline int fused_strips(int v, int n) { return (n + FUSED_TR[v] - 3) / (FUSED_TR[v] - 2); }
+inline int fused_covered(int v, int n, int npad) { return std::min(fused_strips(v, n) * (FUSED_TR[v] - 2), npad); }
 
 template <int ALAY>
 void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool profiled) {
@@ -336,6 +467,8 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
         case 3: go(sweep_fused_kernel_d2<ALAY, 2>); break;
         case 4: go(sweep_fused_kernel<ALAY, 2, 4>); break;
         case 5: go(sweep_fused_kernel<ALAY, 2, 2>); break;
+        case 6: go(sweep_fused_kernel<ALAY, 2, 4, 32>); break;
+        case 7: go(sweep_fused_kernel<ALAY, 3, 4, 32>); break;
         default: go(sweep_fused_kernel_d2<ALAY, 4>); break;
     }
 }
@@ -349,9 +482,18 @@ struct FusedKey {
 std::mutex g_fused_mu;
 std::map<FusedKey, int> g_fused_choice;
 
+bool fused_eligible(int v, const FusedParams &fp) {
+    return (fp.Rpad / 16) % (4 / FUSED_KS[v]) == 0 && FUSED_TR[v] <= fp.npad;
+}
+
 // Times every eligible variant on `fp` (a real step whose outputs the sweep rewrites before
 // reading them) and returns the fastest; the choice is kept per (npad, Rpad, adjoint).
+// WUST_SWEEP_VARIANT=v forces variant v where it is eligible (read per solve, for tests).
 int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
+    if (const char *e = std::getenv("WUST_SWEEP_VARIANT"); e && e[0]) {
+        const int v = std::atoi(e);
+        if (v >= 0 && v < N_FUSED_VARIANTS && fused_eligible(v, fp)) return v;
+    }
     static const bool tune = [] {
         const char *e = std::getenv("WUST_AUTOTUNE");
         return !(e && e[0] == '0');
@@ -378,9 +520,10 @@ int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
     };
     for (int v = 0; v < N_FUSED_VARIANTS; ++v) {
         const int tpw = 4 / FUSED_KS[v];
-        ok[v] = (fp.Rpad / 16) % tpw == 0;
+        ok[v] = fused_eligible(v, fp);
         per_launch[v] = 1e30f;
         q[v] = fp;
+        q[v].strips = fused_strips(v, fp.n);
         q[v].units = q[v].ntasks * q[v].strips * (q[v].Rpad / 16) / tpw;
         q[v].per_xcd = (q[v].units + 7) / 8;
         if (ok[v]) run(v);    // warms the code object and the caches
@@ -401,10 +544,16 @@ int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
     int best = 0;
     for (int v = 0; v < N_FUSED_VARIANTS; ++v)
         if (ok[v] && (!ok[best] || per_launch[v] < per_launch[best])) best = v;
-    if (const char *e = std::getenv("WUST_AUTOTUNE_LOG"); e && e[0] == '1')
-        std::fprintf(stderr, "sweep step kernel: npad %d Rpad %d %s -> depth %d, K split %d%s (%.2f us)\n", fp.npad,
-                     fp.Rpad, adjoint ? "adjoint" : "forward", FUSED_DEPTH[best], FUSED_KS[best],
-                     FUSED_FIVE[best] ? ", five waves" : "", 1e3f * per_launch[best]);
+    if (const char *e = std::getenv("WUST_AUTOTUNE_LOG"); e && e[0] == '1') {
+        std::fprintf(stderr, "sweep step variants, npad %d Rpad %d %s (us per launch):", fp.npad, fp.Rpad,
+                     adjoint ? "adjoint" : "forward");
+        for (int v = 0; v < N_FUSED_VARIANTS; ++v)
+            if (ok[v]) std::fprintf(stderr, " [%d] %.2f", v, 1e3f * per_launch[v]);
+        std::fprintf(stderr, "\n");
+        std::fprintf(stderr, "sweep step kernel: npad %d Rpad %d %s -> %d-row tiles, depth %d, K split %d%s (%.2f us)\n",
+                     fp.npad, fp.Rpad, adjoint ? "adjoint" : "forward", FUSED_TR[best], FUSED_DEPTH[best],
+                     FUSED_KS[best], FUSED_FIVE[best] ? ", five waves" : "", 1e3f * per_launch[best]);
+    }
     std::lock_guard<std::mutex> lk(g_fused_mu);
     g_fused_choice[key] = best;
     return best;
@@ -435,10 +584,11 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
     auto OP = [&](int chain, int pp) { return ops + (int64_t)(2 * chain + pp) * blk; };
     int Ld[3] = {DNW, DW, DSW};
     int Ud[3] = {DNE, DE, DSE};
-    // strips cover the n real rows only (1280 rather than 1344 workgroups at 558 rows and 256 RHS:
-    // five per CU); the padding rows beyond them are zeroed once per solve
-    const int strips = (op.n + FUSED_OWN - 1) / FUSED_OWN;
-    const int covered = std::min(strips * FUSED_OWN, npad);
+    // strips cover the n real rows only (1280 rather than 1344 workgroups at 558 rows, 256 RHS and
+    // 16-row tiles: five per CU); the padding rows beyond what every variant's strips cover are
+    // zeroed once per solve
+    int covered = npad;
+    for (int v = 0; v < N_FUSED_VARIANTS; ++v) covered = std::min(covered, fused_covered(v, op.n, npad));
     if (covered < npad) {
         const int64_t per = (int64_t)(npad - covered) * Rpad, most = per * (m + 4);
         hipLaunchKernelGGL(pad_rows_zero_kernel, dim3((unsigned)((most + 255) / 256), 2), dim3(256), 0, s, work,
@@ -457,12 +607,14 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
         FusedParams fp{};
         fp.ntasks = (int)tasks.size();
         for (int i = 0; i < fp.ntasks; ++i) fp.task[i] = tasks[i];
+        fp.n = op.n;
         fp.npad = npad;
         fp.Rpad = Rpad;
-        fp.strips = strips;
+        fp.strips = fused_strips(0, op.n);
         if (variant < 0) variant = fused_variant(fp, adjoint, s);
         const int tpw = 4 / FUSED_KS[variant];
-        fp.units = fp.ntasks * strips * (Rpad / 16) / tpw;
+        fp.strips = fused_strips(variant, op.n);
+        fp.units = fp.ntasks * fp.strips * (Rpad / 16) / tpw;
         fp.per_xcd = (fp.units + 7) / 8;
         prof_units(PC_SWEEP_GEMM, fp.ntasks);
         if (adjoint) launch_fused_variant<A_CONJT>(variant, fp, s, true);
