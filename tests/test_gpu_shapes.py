@@ -165,10 +165,10 @@ def test_zero_column_skip_edges(monkeypatch, where):
 @pytest.mark.parametrize("adjoint", [False, True])
 def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
     """Every kernel variant of the fused sweep step, forced in turn: the variants that split K over
-    four waves (16- or 32-row tiles, any prefetch depth) accumulate each element in the same order
+    four waves (16-, 32- or 64-row tiles, any prefetch depth) accumulate each element in the same order
     and give bit-identical solutions; those that split K over two waves agree to single-precision
     rounding.  Each solution solves the oracle's H to single-precision accuracy.  (52 rows: the
-    32-row tiles' last strip is clamped to the block end and the padding rows beyond the strips
+    taller tiles' last strip is clamped to the block end and the padding rows beyond the strips
     are zero.)"""
     x, y, c = _grid(ny, nx)
     f, a0, L = 2.2e5, 10.0, 2e-3
@@ -183,7 +183,8 @@ def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
     interior = np.zeros((ny, nx), bool)
     interior[1:-1, 1:-1] = True
     base = None
-    for v, ksplit4 in [(0, True), (1, True), (2, False), (3, False), (4, True), (5, False), (6, True), (7, True)]:
+    for v, ksplit4 in [(0, True), (1, True), (2, False), (3, False), (4, True), (5, False), (6, True), (7, True),
+                      (8, True)]:
         monkeypatch.setenv("WUST_SWEEP_VARIANT", str(v))
         u = s.solve(b, adjoint)[0].clone()
         if base is None:

@@ -439,13 +439,13 @@ __global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r
 
 // Kernel variants of the fused step and their per-shape choice.  Variant v: (DEPTH, KS, tile
 // rows) = (2, 4, 16) five waves, (3, 4, 16), (4, 2, 16), (2, 2, 16) five waves, (2, 4, 16),
-// (2, 2, 16), (2, 4, 32), (3, 4, 32); a variant with TPW tiles per workgroup needs Rpad / 16 to
-// be a multiple of TPW.
-constexpr int N_FUSED_VARIANTS = 8;
-constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2, 4, 2, 4, 4};
-constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {2, 3, 4, 2, 2, 2, 2, 3};
-constexpr bool FUSED_FIVE[N_FUSED_VARIANTS] = {true, false, false, true, false, false, false, false};
-constexpr int FUSED_TR[N_FUSED_VARIANTS] = {16, 16, 16, 16, 16, 16, 32, 32};
+// (2, 2, 16), (2, 4, 32), (3, 4, 32), (2, 4, 64); a variant with TPW tiles per workgroup needs
+// Rpad / 16 to be a multiple of TPW.
+constexpr int N_FUSED_VARIANTS = 9;
+constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2, 4, 2, 4, 4, 4};
+constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {2, 3, 4, 2, 2, 2, 2, 3, 2};
+constexpr bool FUSED_FIVE[N_FUSED_VARIANTS] = {true, false, false, true, false, false, false, false, false};
+constexpr int FUSED_TR[N_FUSED_VARIANTS] = {16, 16, 16, 16, 16, 16, 32, 32, 64};
 
 // strips of variant v's tiles over n rows (each owns FUSED_TR - 2 rows), and the rows they cover
 inline int fused_strips(int v, int n) { return (n + FUSED_TR[v] - 3) / (FUSED_TR[v] - 2); }
@@ -469,6 +469,7 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
         case 5: go(sweep_fused_kernel<ALAY, 2, 2>); break;
         case 6: go(sweep_fused_kernel<ALAY, 2, 4, 32>); break;
         case 7: go(sweep_fused_kernel<ALAY, 3, 4, 32>); break;
+        case 8: go(sweep_fused_kernel<ALAY, 2, 4, 64>); break;
         default: go(sweep_fused_kernel_d2<ALAY, 4>); break;
     }
 }
