@@ -19,6 +19,7 @@ struct ResidualArgs {
     double *norm2;        // per-RHS sum |res|^2 accumulated with atomics (may be null)
     double *bnorm2;       // per-RHS sum |b|^2 accumulated with atomics (may be null)
     int n, m, npad, Rpad, adjoint;
+    int cols;             // columns per workgroup (set by launch_residual)
 };
 
 struct SourceFitArgs {

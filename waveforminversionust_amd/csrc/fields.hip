@@ -90,7 +90,7 @@ __constant__ int c_dx[9] = {0, 0, 0, -1, -1, -1, 1, 1, 1};
 __constant__ int c_opp[9] = {DC, DS, DN, DE, DSE, DNE, DW, DSW, DNW};
 
 // One thread per (RHS r, row y), a workgroup per 16 RHS x 16 rows that marches along a run of
-// RES_COLS columns.  The iterate values of columns x-1, x, x+1 over the tile's rows and one halo
+// a.cols <= RES_COLS columns.  The iterate values of columns x-1, x, x+1 over the tile's rows and one halo
 // row on each side sit in an LDS ring of four column slots: a column step writes the column that
 // the previous step loaded (each thread its own row, 32 threads the halo rows), issues the loads
 // of the column after it, and after one barrier reads its nine neighbours from the ring, so every
@@ -102,7 +102,7 @@ __constant__ int c_opp[9] = {DC, DS, DN, DE, DSE, DNE, DW, DSW, DNW};
 // u += d; per-RHS |r|^2 and |b|^2 are summed over the run in registers and reduced per workgroup
 // in LDS before the atomics.  The iterate is U_old (c128), D0 (c64: the first correction, kept in
 // single precision so that the first pass stores no FP64 iterate) or zero.
-constexpr int RES_COLS = 32;
+constexpr int RES_COLS = 32;   // longest run; shorter runs keep small grids' workgroup counts up
 
 template <bool HAS_U, bool HAS_D0, bool HAS_D>
 __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
     const int r = blockIdx.x * 16 + lr;
     const int y0 = blockIdx.y * 16, yi = y0 + ly;
     const int n = a.n, m = a.m, npad = a.npad, Rpad = a.Rpad;
-    const int xa = blockIdx.z * RES_COLS, xb = min(xa + RES_COLS, m);
+    const int xa = blockIdx.z * a.cols, xb = min(xa + a.cols, m);
     const int64_t colstride = (int64_t)npad * Rpad;
     const bool live = yi < n;
     // ring rows j = 0 .. 17 hold grid rows y0 - 1 + j, clamped to the interior; thread t holds its
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
         }
     };
 
-    for (int e = threadIdx.x; e < RES_COLS * 16; e += 256) {
+    for (int e = threadIdx.x; e < a.cols * 16; e += 256) {
         // mask byte of column xa + e / 16, tile row e % 16 (the 16 RHS share one byte)
         const int bx = xa + (e >> 4), brow = min(y0 + (e & 15), n - 1);
         uint8_t v = 1;
@@ -467,15 +467,19 @@ void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int 
     WUST_LAUNCH_CHECK();
 }
 void launch_residual(const ResidualArgs &a, hipStream_t s) {
-    dim3 grid(a.Rpad / 16, a.npad / 16, (a.m + RES_COLS - 1) / RES_COLS);
+    // runs of up to RES_COLS columns, as long as at least 4096 workgroups remain
+    const int64_t tiles = (int64_t)(a.Rpad / 16) * (a.npad / 16) * a.m;
+    ResidualArgs k = a;
+    k.cols = (int)std::max<int64_t>(1, std::min<int64_t>(RES_COLS, tiles / 4096));
+    dim3 grid(a.Rpad / 16, a.npad / 16, (a.m + k.cols - 1) / k.cols);
     require(!(a.u_old && a.d0), "residual: U_old and D0 are exclusive");
     require(a.Rpad % 16 == 0 && a.npad % 16 == 0 && a.n >= 1 && a.n <= a.npad, "residual: padded sizes");
-    if (a.u_old && a.d32) launch_k(PC_RESIDUAL, residual_kernel<true, false, true>, grid, dim3(256), 0, s, a);
-    else if (a.u_old) launch_k(PC_RESIDUAL, residual_kernel<true, false, false>, grid, dim3(256), 0, s, a);
-    else if (a.d0 && a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, true, true>, grid, dim3(256), 0, s, a);
-    else if (a.d0) launch_k(PC_RESIDUAL, residual_kernel<false, true, false>, grid, dim3(256), 0, s, a);
-    else if (a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, false, true>, grid, dim3(256), 0, s, a);
-    else launch_k(PC_RESIDUAL, residual_kernel<false, false, false>, grid, dim3(256), 0, s, a);
+    if (a.u_old && a.d32) launch_k(PC_RESIDUAL, residual_kernel<true, false, true>, grid, dim3(256), 0, s, k);
+    else if (a.u_old) launch_k(PC_RESIDUAL, residual_kernel<true, false, false>, grid, dim3(256), 0, s, k);
+    else if (a.d0 && a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, true, true>, grid, dim3(256), 0, s, k);
+    else if (a.d0) launch_k(PC_RESIDUAL, residual_kernel<false, true, false>, grid, dim3(256), 0, s, k);
+    else if (a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, false, true>, grid, dim3(256), 0, s, k);
+    else launch_k(PC_RESIDUAL, residual_kernel<false, false, false>, grid, dim3(256), 0, s, k);
 }
 void promote(const cf *in, cd *out, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(promote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
