@@ -178,6 +178,7 @@ struct wust_helmholtz {
     PlainLib plain;
     int ws_Rpad = 0;
     cd *b64 = nullptr, *t64 = nullptr, *u64 = nullptr, *lam64 = nullptr;
+    bool b64_zero = false;      // b64 holds only zeros (the oracle evaluation undoes what it scatters)
     cf *s32 = nullptr, *d32 = nullptr, *d32b = nullptr, *b32 = nullptr;
     uint8_t *bmask = nullptr;   // per 64 values of the current right-hand side: any nonzero
     bool bmask_valid = false;
@@ -266,6 +267,7 @@ struct wust_helmholtz {
         release_ws();
         int64_t cnt = field_elems(Rpad);
         b64 = dalloc<cd>(cnt);
+        b64_zero = false;
         t64 = dalloc<cd>(cnt);
         u64 = dalloc<cd>(cnt);
         lam64 = dalloc<cd>(cnt);
@@ -633,6 +635,7 @@ int wust_helmholtz_solve(wust_helmholtz *h, const void *d_src, void *d_out, int6
         WUST_HIP_CHECK(hipEventRecord(e0, s));
         const int Rpad = (int)round_up(R, 16);
         h->ensure_ws(Rpad);
+        h->b64_zero = false;
         ext_to_solve64((const cd *)d_src, h->b64, h->op.ny, h->op.nx, h->op.npad, (int)R, Rpad, nullptr, s);
         wust_solve_stats st =
             h->solve_internal(h->b64, h->u64, (int)R, Rpad, adjoint != 0, tol, max_iter, stats != nullptr, s);
@@ -700,8 +703,10 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
             const int Rpad = (int)round_up(nb, 16);
             h->ensure_ws(Rpad);
             const int64_t cnt = h->field_elems(Rpad);
-            // unit impulses at the batch's transmitters (Runtime/matlab/wust_oracle.m:31-34)
-            WUST_HIP_CHECK(hipMemsetAsync(h->b64, 0, sizeof(cd) * cnt, s));
+            // unit impulses at the batch's transmitters (Runtime/matlab/wust_oracle.m:31-34), into b64
+            // kept zero between evaluations: each scatter below is undone after its solve
+            if (!h->b64_zero) WUST_HIP_CHECK(hipMemsetAsync(h->b64, 0, sizeof(cd) * cnt, s));
+            h->b64_zero = false;
             launch_impulses(h->b64, txx + first, txy + first, nb, op.npad, Rpad, s);
             cd *ufwd = h->u64;
             if (retain) ufwd = dalloc<cd>(cnt);
@@ -709,8 +714,8 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
                 h->solve_internal(h->b64, ufwd, nb, Rpad, false, tol, max_iter, false, s);
                 // source fit, residual, loss and the adjoint source
                 h->clear_status(s);
-                cd *adj = h->b64;  // forward RHS no longer needed
-                WUST_HIP_CHECK(hipMemsetAsync(adj, 0, sizeof(cd) * cnt, s));
+                cd *adj = h->b64;  // forward RHS no longer needed: undo its impulses
+                launch_impulses(adj, txx + first, txy + first, nb, op.npad, Rpad, s, 0.0);
                 SourceFitArgs fa{};
                 fa.u = ufwd;
                 fa.Y = (const cd *)d_Y;
@@ -756,6 +761,9 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
                     ga.R = nb;
                     launch_gradient(ga, s);
                 }
+                // undo the adjoint source (written at the receivers' nodes for RHS r < nb)
+                launch_clear_points(adj, rxx, rxy, (int)n_rx, nb, op.npad, Rpad, s);
+                h->b64_zero = true;
                 if (retain) {
                     Retained rt;
                     rt.u = ufwd;
@@ -802,6 +810,7 @@ int wust_directional_curvature(wust_helmholtz *h, const double *d_direction, con
             ca.u = rt.u;
             ca.vsm = op.vsm;
             ca.direction = d_direction;
+            h->b64_zero = false;
             ca.src = h->b64;
             ca.d = op.d;
             ca.e = op.e;

@@ -71,7 +71,11 @@ void demote(const cd *in, cf *out, int64_t count, hipStream_t s);
 // (flags pre-zeroed; per a multiple of 64); mask[i / 64] = whether values 64 (i / 64) .. + 63 hold
 // a nonzero (count a multiple of 64)
 void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, uint8_t *mask, hipStream_t s);
-void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s);
+// b at node (sx[r], sy[r]) of RHS r = value, r < R (value 0 undoes the impulses)
+void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s,
+                     double value = 1.0);
+// b = 0 at nodes (px[q], py[q]) for every RHS r < R (px[q] < 0: no node)
+void launch_clear_points(cd *b, const int *px, const int *py, int npts, int R, int npad, int Rpad, hipStream_t s);
 void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s);
 void launch_gradient(const GradientArgs &a, hipStream_t s);
 void launch_curvature_source(const CurvSrcArgs &a, hipStream_t s);

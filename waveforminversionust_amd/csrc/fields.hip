@@ -273,10 +273,18 @@ __global__ void demote_flag_kernel(const cd *__restrict__ in, cf *__restrict__ o
 
 // unit impulses: B[xi][yi][r] = 1 at the source node of RHS r (B pre-zeroed)
 __global__ void impulse_kernel(cd *__restrict__ b, const int *__restrict__ sx, const int *__restrict__ sy, int R,
-                               int npad, int Rpad) {
+                               int npad, int Rpad, double value) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
-    b[((int64_t)sx[r] * npad + sy[r]) * Rpad + r] = mkd(1, 0);
+    b[((int64_t)sx[r] * npad + sy[r]) * Rpad + r] = mkd(value, 0);
+}
+
+// zeroes b at nodes (px[q], py[q]) for every RHS r < R (px[q] < 0: no node)
+__global__ void clear_points_kernel(cd *__restrict__ b, const int *__restrict__ px, const int *__restrict__ py,
+                                    int npts, int R, int npad, int Rpad) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (r >= R || q >= npts || px[q] < 0) return;
+    b[((int64_t)px[q] * npad + py[q]) * Rpad + r] = mkd(0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -495,8 +503,14 @@ void demote(const cd *in, cf *out, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(demote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
     WUST_LAUNCH_CHECK();
 }
-void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s) {
-    hipLaunchKernelGGL(impulse_kernel, dim3((R + 255) / 256), dim3(256), 0, s, b, sx, sy, R, npad, Rpad);
+void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s, double value) {
+    hipLaunchKernelGGL(impulse_kernel, dim3((R + 255) / 256), dim3(256), 0, s, b, sx, sy, R, npad, Rpad, value);
+    WUST_LAUNCH_CHECK();
+}
+void launch_clear_points(cd *b, const int *px, const int *py, int npts, int R, int npad, int Rpad, hipStream_t s) {
+    if (npts <= 0 || R <= 0) return;
+    hipLaunchKernelGGL(clear_points_kernel, dim3((R + 255) / 256, npts), dim3(256), 0, s, b, px, py, npts, R, npad,
+                       Rpad);
     WUST_LAUNCH_CHECK();
 }
 void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s) {
