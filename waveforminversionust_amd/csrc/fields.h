@@ -41,6 +41,7 @@ struct GradientArgs {
     double *grad;         // [Ny*Nx], accumulated
     double d, e;
     int ny, n, m, npad, Rpad, R;
+    int cols;             // columns per workgroup (set by launch_gradient)
 };
 
 struct CurvSrcArgs {

@@ -351,30 +351,85 @@ __global__ void __launch_bounds__(256) source_fit_kernel(SourceFitArgs a) {
 // g(y,x) += sum_r -Re( conj(a_r vsm u) .* (K lambda) ),  K = mass stencil,
 // lambda = 0 on the Dirichlet ring (massApply(., true) zeroes the input border)
 // ---------------------------------------------------------------------------
+// One workgroup per 8 rows x a run of a.cols <= GRAD_COLS columns, 32 RHS lanes per row; the
+// lambda values of columns x-1, x, x+1 over the rows and one halo row on each side sit in an LDS
+// ring of four column slots (zero outside the interior), filled one column per step as in the
+// residual, so each lambda value is loaded once per workgroup instead of nine times.  The RHS run
+// in chunks of 32; a node's sum over a chunk's lanes is a shuffle tree, added in chunk order to the
+// node's accumulator in LDS, so every node is summed by one workgroup in a fixed order (no atomics).
+constexpr int GRAD_COLS = 32;
+
 __global__ void __launch_bounds__(256) gradient_kernel(GradientArgs a) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int yi = blockIdx.x * 4 + wave;
-    const int xi = blockIdx.y;
-    if (yi >= a.n) return;
-    const int npad = a.npad, Rpad = a.Rpad;
+    const int lr = threadIdx.x & 31, ly = threadIdx.x >> 5;
+    const int y0 = blockIdx.x * 8, yi = y0 + ly;
+    const int n = a.n, m = a.m, npad = a.npad, Rpad = a.Rpad;
+    const int xa = blockIdx.y * a.cols, xb = min(xa + a.cols, m);
+    const int64_t colstride = (int64_t)npad * Rpad;
     const double kc = 1 - a.d - a.e, ke = a.d / 4, kd = a.e / 4;
-    const cd v = a.vsm[(int64_t)(xi + 1) * a.ny + (yi + 1)];
-    double acc = 0;
-    for (int r = lane; r < a.R; r += 64) {
-        auto L = [&](int yy, int xx) -> cd {
-            if (yy < 0 || yy >= a.n || xx < 0 || xx >= a.m) return mkd(0, 0);
-            return a.lam[((int64_t)xx * npad + yy) * Rpad + r];
+    // ring rows j = 0 .. 9 hold grid rows y0 - 1 + j: thread t its own row (j = t / 32 + 1), threads
+    // t < 64 also the halo rows j = 0 (t < 32) and j = 9
+    const bool halo = threadIdx.x < 64;
+    const int hj = threadIdx.x < 32 ? 0 : 9, hy = threadIdx.x < 32 ? y0 - 1 : y0 + 8;
+    const bool own_in = yi < n, halo_in = halo && hy >= 0 && hy < n;
+    const int own_y = min(yi, n - 1), halo_y = min(max(hy, 0), n - 1);
+    __shared__ cd ring[4][10][32];
+    __shared__ double acc[GRAD_COLS][8];
+    for (int e = threadIdx.x; e < GRAD_COLS * 8; e += 256) acc[e >> 3][e & 7] = 0;
+    for (int r0 = 0; r0 < a.R; r0 += 32) {
+        const int r = r0 + lr;
+        const bool rin = r < a.R;
+        const int rc = min(r, Rpad - 1);              // in-range address for lanes beyond the RHS
+        const cd sc = rin ? a.scale[r] : mkd(0, 0);
+        cd pl = mkd(0, 0), hl = mkd(0, 0);            // loaded column: own row, halo row
+        auto load_col = [&](int x) {
+            const bool xin = x >= 0 && x < m;
+            const int64_t base = (int64_t)min(max(x, 0), m - 1) * colstride + rc;
+            pl = a.lam[base + (int64_t)own_y * Rpad];
+            if (!(xin && own_in && rin)) pl = mkd(0, 0);
+            if (halo) {
+                hl = a.lam[base + (int64_t)halo_y * Rpad];
+                if (!(xin && halo_in && rin)) hl = mkd(0, 0);
+            }
         };
-        cd kl = kc * L(yi, xi);
-        kl = kl + ke * (L(yi - 1, xi) + L(yi + 1, xi) + L(yi, xi - 1) + L(yi, xi + 1));
-        kl = kl + kd * (L(yi - 1, xi - 1) + L(yi + 1, xi - 1) + L(yi - 1, xi + 1) + L(yi + 1, xi + 1));
-        cd u = a.u[((int64_t)xi * npad + yi) * Rpad + r];
-        cd av = a.scale[r] * (v * u);
-        // -Re(conj(av) * kl)
-        acc -= av.re * kl.re + av.im * kl.im;
+        auto store_col = [&](int x) {
+            const int sl = x & 3;
+            ring[sl][ly + 1][lr] = pl;
+            if (halo) ring[sl][hj][lr] = hl;
+        };
+        auto load_u = [&](int x) { return a.u[(int64_t)min(x, m - 1) * colstride + (int64_t)own_y * Rpad + rc]; };
+        auto load_v = [&](int x) { return a.vsm[(int64_t)(min(x, m - 1) + 1) * a.ny + (own_y + 1)]; };
+        __syncthreads();                              // the previous chunk's last reads of the ring
+        load_col(xa - 1);
+        store_col(xa - 1);
+        load_col(xa);
+        store_col(xa);
+        load_col(xa + 1);
+        cd un = load_u(xa), vn = load_v(xa);
+#pragma unroll 1
+        for (int x = xa; x < xb; ++x) {
+            store_col(x + 1);
+            const cd u = un, v = vn;
+            load_col(x + 2);
+            un = load_u(x + 1);
+            vn = load_v(x + 1);
+            __syncthreads();
+            auto L = [&](int dy, int dx) { return ring[(x + dx) & 3][ly + 1 + dy][lr]; };
+            // the reference's order: centre, the four edge neighbours, the four corners
+            cd kl = kc * L(0, 0);
+            kl = kl + ke * (L(-1, 0) + L(1, 0) + L(0, -1) + L(0, 1));
+            kl = kl + kd * (L(-1, -1) + L(1, -1) + L(-1, 1) + L(1, 1));
+            const cd av = sc * (v * u);
+            double c = rin ? -(av.re * kl.re + av.im * kl.im) : 0.0;   // -Re(conj(av) kl)
+#pragma unroll
+            for (int off = 16; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+            if (lr == 0) acc[x - xa][ly] += c;
+        }
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) a.grad[(int64_t)(xi + 1) * a.ny + (yi + 1)] += acc;
+    __syncthreads();
+    for (int e = threadIdx.x; e < GRAD_COLS * 8; e += 256) {
+        const int x = xa + (e >> 3), y = y0 + (e & 7);
+        if (x < xb && y < n) a.grad[(int64_t)(x + 1) * a.ny + (y + 1)] += acc[e >> 3][e & 7];
+    }
 }
 
 // curvature source: src(i, r) = sum_j K(j - i) vsm(j) u(j, r) delta(j), j interior
@@ -517,8 +572,13 @@ void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s) {
     launch_k(PC_SOURCE_FIT, source_fit_kernel, dim3(R), dim3(256), 0, s, a);
 }
 void launch_gradient(const GradientArgs &a, hipStream_t s) {
-    dim3 grid((a.n + 3) / 4, a.m);
-    launch_k(PC_GRADIENT, gradient_kernel, grid, dim3(256), 0, s, a);
+    // runs of up to GRAD_COLS columns, as long as at least 2048 workgroups remain
+    require(a.Rpad % 16 == 0 && a.npad % 8 == 0 && a.n >= 1 && a.R <= a.Rpad, "gradient: padded sizes");
+    GradientArgs k = a;
+    const int64_t tiles = (int64_t)(a.npad / 8) * a.m;
+    k.cols = (int)std::max<int64_t>(1, std::min<int64_t>(GRAD_COLS, tiles / 2048));
+    dim3 grid(a.npad / 8, (a.m + k.cols - 1) / k.cols);
+    launch_k(PC_GRADIENT, gradient_kernel, grid, dim3(256), 0, s, k);
 }
 void launch_curvature_source(const CurvSrcArgs &a, hipStream_t s) {
     dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);
