@@ -72,6 +72,10 @@ void demote(const cd *in, cf *out, int64_t count, hipStream_t s);
 // (flags pre-zeroed; per a multiple of 64); mask[i / 64] = whether values 64 (i / 64) .. + 63 hold
 // a nonzero (count a multiple of 64)
 void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, uint8_t *mask, hipStream_t s);
+// the same for a right-hand side whose nonzeros all lie at the nodes (px[q], py[q]) (px < 0: none),
+// reading only those nodes (flags pre-zeroed; count a multiple of 64)
+void sparse_demote_flag(const cd *in, cf *out, int64_t count, const int *px, const int *py, int npts, int npad,
+                        int Rpad, int *flags, uint8_t *mask, hipStream_t s);
 // b at node (sx[r], sy[r]) of RHS r = value, r < R (value 0 undoes the impulses)
 void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s,
                      double value = 1.0);

@@ -271,6 +271,23 @@ __global__ void demote_flag_kernel(const cd *__restrict__ in, cf *__restrict__ o
     }
 }
 
+// demote_flag for a right-hand side whose nonzeros all lie at the nodes (px[q], py[q]): converts
+// those nodes' values for every RHS (out and mask pre-zeroed) and flags the block columns and
+// 64-value chunks that hold a nonzero, as demote_flag does over the whole field
+__global__ void sparse_demote_flag_kernel(const cd *__restrict__ in, cf *__restrict__ out, const int *__restrict__ px,
+                                          const int *__restrict__ py, int npts, int npad, int Rpad,
+                                          int *__restrict__ flags, uint8_t *__restrict__ mask) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (r >= Rpad || q >= npts || px[q] < 0) return;
+    const int64_t i = ((int64_t)px[q] * npad + py[q]) * Rpad + r;
+    const cd v = in[i];
+    out[i] = tof(v);
+    if (v.re != 0.0 || v.im != 0.0) {
+        flags[px[q]] = 1;
+        mask[i >> 6] = 1;
+    }
+}
+
 // unit impulses: B[xi][yi][r] = 1 at the source node of RHS r (B pre-zeroed)
 __global__ void impulse_kernel(cd *__restrict__ b, const int *__restrict__ sx, const int *__restrict__ sy, int R,
                                int npad, int Rpad, double value) {
@@ -560,6 +577,16 @@ void demote(const cd *in, cf *out, int64_t count, hipStream_t s) {
 }
 void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s, double value) {
     hipLaunchKernelGGL(impulse_kernel, dim3((R + 255) / 256), dim3(256), 0, s, b, sx, sy, R, npad, Rpad, value);
+    WUST_LAUNCH_CHECK();
+}
+void sparse_demote_flag(const cd *in, cf *out, int64_t count, const int *px, const int *py, int npts, int npad,
+                        int Rpad, int *flags, uint8_t *mask, hipStream_t s) {
+    require(count % 64 == 0, "sparse_demote_flag: size must be a multiple of 64");
+    WUST_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(cf) * count, s));
+    WUST_HIP_CHECK(hipMemsetAsync(mask, 0, count / 64, s));
+    if (npts <= 0) return;
+    hipLaunchKernelGGL(sparse_demote_flag_kernel, dim3((Rpad + 255) / 256, npts), dim3(256), 0, s, in, out, px, py,
+                       npts, npad, Rpad, flags, mask);
     WUST_LAUNCH_CHECK();
 }
 void launch_clear_points(cd *b, const int *px, const int *py, int npts, int R, int npad, int Rpad, hipStream_t s) {
