@@ -840,9 +840,14 @@ void factor_engine(Operator &op, hipStream_t s) {
             for (int i = 0; i < nsteps && i < 48; ++i) std::fprintf(stderr, " %d", (int)next[i]);
             std::fprintf(stderr, " ...\n");
         }
+        // An unconverged factorization keeps its profile too: the steps whose blocks ran out of
+        // rounds (forced, one more than they had) or needed 4+ reach Gauss-Jordan in the next
+        // factorization with this key, which seeds the later steps from correct inverses.  (Starting
+        // again from the defaults would repeat a divergence that the defaults cause, e.g. the third
+        // column of a chain at low frequency, and the full Gauss-Jordan fallback, at every
+        // factorization.)
         std::lock_guard<std::mutex> lk(g_budget_mu);
-        if (st[0] & ST_INVERSE_UNCONVERGED) g_budgets.erase(key);
-        else g_budgets[key] = next;
+        g_budgets[key] = next;
     }
     op.factor_fallback = 0;
     if (st[0] & ST_INVERSE_UNCONVERGED) {
