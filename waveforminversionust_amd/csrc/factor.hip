@@ -648,14 +648,17 @@ struct FactorRun {
 
     // Newton-Schulz round budget of chain step `st`: the PML columns at the
     // start of each chain change fastest.
-    // Newton-Schulz or Gauss-Jordan for chain step `st`.  A Gauss-Jordan block costs about as
-    // much as 3-4 Newton-Schulz rounds (npad / 32 panel launches against three GEMM launches per
-    // round), so steps that needed 4 or more rounds, or whose need is unknown, use Gauss-Jordan;
+    // Newton-Schulz or Gauss-Jordan for chain step `st`.  A Gauss-Jordan block (npad / 32 panel
+    // launches) costs more than 3 Newton-Schulz rounds (three GEMM launches each) at npad 576
+    // (0.31 against 0.26 ms for both chains) and less at npad 1088 (0.84 against 1.08 ms), the
+    // GEMMs growing as npad^3 and the latency-bound panels much more slowly.  So steps that needed
+    // 4 or more rounds (3 or more from npad 800 on), or whose need is unknown, use Gauss-Jordan;
     // without a learned profile the second column of each chain (seeded from the first, the
     // slowest to converge: 7-9 rounds at 6 points per wavelength) does too.
     bool ns_step(int st) const {
         if (learned.empty()) return st >= 2;
-        return st < (int)learned.size() && learned[st] >= 1 && learned[st] <= 3;
+        const int most = op.npad >= 800 ? 2 : 3;
+        return st < (int)learned.size() && learned[st] >= 1 && learned[st] <= most;
     }
 
     int ns_budget(int st) const {
