@@ -2,8 +2,8 @@
 
 Blocks with an inverted neighbour in their chain are inverted by Newton-Schulz
 iteration (factor='auto') except where Gauss-Jordan is cheaper (the second
-column of each chain, and columns that needed 4 or more rounds in an earlier
-factorization of the same operator key); the chain starts, the meeting block,
+column of each chain, and columns that needed 4 or more rounds, 3 or more from
+npad 800 on, in an earlier factorization of the same operator key); the chain starts, the meeting block,
 and the whole factor when an iteration misses its residual bound use
 Gauss-Jordan elimination (factor='gauss-jordan' forces it everywhere).
 Tolerances:
@@ -95,3 +95,36 @@ def test_newton_schulz_fallback_is_gauss_jordan(ring, monkeypatch):
     u_fb, _ = s_fb.solve(src, False)
     u_gj, _ = s_gj.solve(src, False)
     assert torch.equal(u_fb, u_gj)
+
+
+def test_unconverged_factor_keeps_its_profile(ring, monkeypatch):
+    # An operator key (its own frequency, so that no earlier test taught it) whose Newton-Schulz
+    # blocks cannot converge under a one-update cap: the first factorization falls back to
+    # Gauss-Jordan, and the profile it leaves moves the blocks that ran out of rounds to
+    # Gauss-Jordan step by step, so a later factorization with the same key and cap converges
+    # without the fallback (starting again from the default budgets would fall back every time).
+    f = ring.frequencies_hz[0] * 1.0371
+    monkeypatch.setenv("WUST_NS_MAX_UPDATES", "1")
+
+    def make():
+        return G.HelmholtzSolver(ring.x_m, ring.y_m, ring.c_true, None, f, -1, ring.cfg["pml_strength"],
+                                 ring.cfg["pml_m"], dict(OPTS, mode="c64", factor="auto"))
+
+    infos = [make().info()]
+    assert infos[0]["factor_fallback"] == 1
+    s = None
+    for _ in range(6):
+        s = make()
+        infos.append(s.info())
+        if infos[-1]["factor_fallback"] == 0:
+            break
+    assert infos[-1]["factor_fallback"] == 0, [i["factor_fallback"] for i in infos]
+    assert infos[-1]["inverse_residual"] <= 1e-5   # (under this cap every block may end on Gauss-Jordan)
+    monkeypatch.delenv("WUST_NS_MAX_UPDATES")
+    s_gj = G.HelmholtzSolver(ring.x_m, ring.y_m, ring.c_true, None, f, -1, ring.cfg["pml_strength"],
+                             ring.cfg["pml_m"], dict(OPTS, mode="c64", factor="gauss-jordan"))
+    src = _sources(ring, 8)
+    u, _ = s.solve(src, False)
+    u_gj, _ = s_gj.solve(src, False)
+    assert _rel(u, u_gj) <= 1e-5
+

@@ -830,6 +830,10 @@ void factor_engine(Operator &op, hipStream_t s) {
             const int stp = j < op.p ? j : m - 1 - j;
             const bool forced = used[j] >= 64;
             int need = (used[j] & 63) + (forced ? 1 : 0);
+            // a step that ran out of rounds needs at least one more than its profile gave it, even
+            // when a cap held its rounds below that
+            if (forced && stp < (int)run.learned.size() && run.learned[stp] >= 1)
+                need = std::max<int>(need, run.learned[stp] + 1);
             need = std::min(need, NS_KMAX);
             next[stp] = (uint8_t)std::max<int>(next[stp], need);
         }
