@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WUST_ABI_VERSION 2
+#define WUST_ABI_VERSION 3
 
 typedef enum {
     WUST_OK = 0,
@@ -93,7 +93,10 @@ typedef struct {
     int64_t factor_bytes;          /* device bytes held by the factors */
     double inverse_residual;       /* largest bound on ||I - T_j X_j||_F over Newton-Schulz blocks */
     int32_t iterative_blocks;      /* Schur blocks inverted by Newton-Schulz */
-    int32_t factor_fallback;       /* 1: Newton-Schulz missed its bound, factor redone by Gauss-Jordan */
+    int32_t factor_fallback;       /* 1: Newton-Schulz missed its bound, factor redone by Gauss-Jordan;
+                                      2: a pivot-free Gauss-Jordan block failed its residual check (or met a
+                                         zero pivot), factor redone with partial pivoting (rocSOLVER getrf) */
+    double gj_residual;            /* largest ||I - T_j X_j||_F over the Gauss-Jordan blocks */
 } wust_info;
 
 typedef struct {

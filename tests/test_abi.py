@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(lib, name), f"{name} missing from {N.library_path()}"
         assert name in N.EXPORTS, f"{name} has no ctypes prototype"
-    assert lib.wust_abi_version() == 2
+    assert lib.wust_abi_version() == 3
 
 
 def test_status_ids_match_reference_identifiers():

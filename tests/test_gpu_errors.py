@@ -1,0 +1,106 @@
+"""Error paths and the factor's pivoting guard on the GPU.
+
+* ``WUST:NotConverged``: refinement asked for a residual that FP64 cannot reach in one correction.
+* ``WUST:SingularSchurBlock``: an operator whose complex-FP32 Schur blocks are exactly zero (every
+  coefficient below the smallest FP32 subnormal) is singular to partial-pivot getrf, which is the
+  reference's test (Runtime/solver/decompBlockLU.cu:339-342,373-374; getrf's info > 0).
+* The pivot-free Gauss-Jordan blocks are checked by ||I - T_j X_j||_F; forcing that check to fail
+  (``WUST_GJ_ACCEPT``) redoes the factor with pivoted rocSOLVER inverses, whose refined solves must
+  still meet the FP64 backward-error bound in the oracle's own H.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import helmholtz_fp64 as H  # noqa: E402
+from waveforminversionust_amd import helmholtz as G  # noqa: E402
+from waveforminversionust_amd._native import WustError  # noqa: E402
+
+OPTS = dict(backend="gpu", stencil_bounds=[1400, 1700])
+
+
+def _grid(ny, nx, dx=5e-4):
+    x = (np.arange(nx) - nx / 2) * dx
+    y = (np.arange(ny) - ny / 2) * dx
+    X, Y = np.meshgrid(x, y)
+    c = 1500 + 40 * np.exp(-(X ** 2 + Y ** 2) / (0.2 * (nx * dx) ** 2))
+    return x, y, c
+
+
+def _rhs(ny, nx, R, seed):
+    rng = np.random.default_rng(seed)
+    b = np.zeros((ny, nx, R), np.complex128)
+    b[1:-1, 1:-1] = rng.standard_normal((ny - 2, nx - 2, R)) + 1j * rng.standard_normal((ny - 2, nx - 2, R))
+    return b
+
+
+def test_not_converged():
+    """tol 1e-20 with one correction allowed: WUST:NotConverged (FP64 residuals stop near 1e-16)."""
+    x, y, c = _grid(40, 23)
+    s = G.HelmholtzSolver(x, y, c, None, 2.2e5, -1, 10.0, 2e-3, OPTS)
+    b = _rhs(40, 23, 3, 1)
+    with pytest.raises(WustError) as ei:
+        s.solve(b, False, tol=1e-20, max_iter=1)
+    assert ei.value.identifier == "WUST:NotConverged"
+    # the same handle still solves to a reachable tolerance afterwards
+    u, _, st = s.solve(b, False, tol=1e-12, max_iter=30, return_stats=True)
+    assert st["final_residual"] <= 1e-12
+
+
+@pytest.mark.parametrize("n", [3, 5, 8])
+def test_singular_schur_block(n):
+    """Every FP32 coefficient underflows to zero (spacing 1e30 m, 1e-30 Hz: coefficients ~1e-60 in FP64),
+    so every Schur block is the zero matrix: the pivot-free elimination meets a zero pivot, the pivoted
+    refactorization's getrf reports it, and creation fails with WUST:SingularSchurBlock."""
+    h = 1e30
+    x = (np.arange(n) - n // 2) * h
+    c = np.full((n, n), 1500.0)
+    with pytest.raises(WustError) as ei:
+        G.HelmholtzSolver(x, x.copy(), c, None, 1e-30, -1, 10.0, h, OPTS)
+    assert ei.value.identifier == "WUST:SingularSchurBlock"
+    # the same operator in FP64 is regular: the oracle's sparse LU factors it
+    ref = H.HelmholtzFP64(x, x.copy(), c, np.zeros_like(c), 1e-30, -1, 10.0, h, dict(stencil_bounds=[1400, 1700]))
+    assert np.all(np.isfinite(ref.solve(_rhs(n, n, 1, 2))[0]))
+
+
+def test_gauss_jordan_blocks_checked():
+    """A regular operator: every Gauss-Jordan block passes its check (||I - T X||_F small), so there is no
+    pivoted refactorization (a first factorization may still redo its Newton-Schulz blocks by
+    Gauss-Jordan, factor_fallback 1)."""
+    x, y, c = _grid(66, 40)
+    s = G.HelmholtzSolver(x, y, c, None, 2.2e5, -1, 10.0, 2e-3, OPTS)
+    info = s.info()
+    assert info["factor_fallback"] in (0, 1)
+    assert 0 < info["gj_residual"] <= 1e-3
+
+
+@pytest.mark.parametrize("ny,nx", [(66, 40), (40, 23), (9, 11)])
+@pytest.mark.parametrize("factor", ["auto", "gauss-jordan"])
+def test_pivoted_refactorization(monkeypatch, ny, nx, factor):
+    """With the Gauss-Jordan acceptance forced below any attainable residual, the factor is redone with
+    pivoted getrf inverses (factor_fallback 2); refined forward and adjoint solves still reach an FP64
+    backward error <= 1e-12 in the oracle's H, and the c64 solution agrees with the pivot-free one to
+    single-precision accuracy."""
+    x, y, c = _grid(ny, nx)
+    f, a0, L = 2.2e5, 10.0, 2e-3
+    R = 5
+    b = _rhs(ny, nx, R, ny * nx)
+    ref = H.HelmholtzFP64(x, y, c, np.zeros_like(c), f, -1, a0, L, dict(stencil_bounds=[1400, 1700]))
+    base = G.HelmholtzSolver(x, y, c, None, f, -1, a0, L, dict(OPTS, mode="c64", factor=factor))
+    u_base = base.solve(b, False)[0].cpu().numpy()
+    monkeypatch.setenv("WUST_GJ_ACCEPT", "1e-30")
+    s = G.HelmholtzSolver(x, y, c, None, f, -1, a0, L, dict(OPTS, factor=factor))
+    assert s.info()["factor_fallback"] == 2
+    s64 = G.HelmholtzSolver(x, y, c, None, f, -1, a0, L, dict(OPTS, mode="c64", factor=factor))
+    u64 = s64.solve(b, False)[0].cpu().numpy()
+    assert np.linalg.norm(u64 - u_base) <= 1e-4 * np.linalg.norm(u_base)
+    interior = np.zeros((ny, nx), bool)
+    interior[1:-1, 1:-1] = True
+    bf = b.reshape(-1, R, order="F")
+    for adjoint in (False, True):
+        u = s.solve(b, adjoint, tol=1e-12)[0].cpu().numpy()
+        Hm = ref.H.conj().T if adjoint else ref.H
+        res = (Hm @ u.reshape(-1, R, order="F") - bf).reshape(ny, nx, R, order="F")[interior]
+        assert np.linalg.norm(res) <= 1e-12 * np.linalg.norm(bf), adjoint

@@ -1,0 +1,106 @@
+"""One operator handle driven through the C ABI in sequences that change the batch layout.
+
+The oracle evaluation keeps its right-hand-side buffer zero between calls (it undoes every scatter)
+and the curvature reads the receiver coordinates the last retaining evaluation stored; a handle
+reused across solves, evaluations of different batch sizes and transmitter counts must give the
+same loss, gradient and curvature as a fresh handle for each call.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from waveforminversionust_amd import _native as N  # noqa: E402
+from waveforminversionust_amd import helmholtz as G  # noqa: E402
+from waveforminversionust_amd import workloads as W  # noqa: E402
+
+from conftest import make_obs  # noqa: E402
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def case():
+    return W.ring_case("ring80_40tx", 64, 8, 0.5e-3, 40, 14e-3, [3e5], "discs")
+
+
+@pytest.fixture(scope="module")
+def obs(case):
+    return make_obs(case)
+
+
+def _solver(case):
+    return G.HelmholtzSolver(case.x_m, case.y_m, np.full((case.ny, case.nx), 1500.0), None,
+                             case.frequencies_hz[0], -1, case.cfg["pml_strength"], case.cfg["pml_m"],
+                             dict(backend="gpu", stencil_bounds=case.cfg["stencil_bounds"], tol=TOL))
+
+
+def _eval(s, obs, tx_sel, rx_sel, batch, retain=False):
+    """wust_oracle_eval on an existing handle for the transmitters tx_sel and receivers rx_sel."""
+    dev = s.device
+    Y = np.asarray(obs["Y"])[np.ix_(tx_sel, rx_sel, [0])][:, :, 0]
+    Yd = torch.as_tensor(Y, device=dev).to(torch.complex128).t().contiguous()
+    Md = torch.ones((len(rx_sel), len(tx_sel)), dtype=torch.uint8, device=dev)
+    tx = np.ascontiguousarray(np.asarray(obs["tx_index"], np.int64)[tx_sel])
+    rx = np.ascontiguousarray(np.asarray(obs["rx_index"], np.int64)[rx_sel])
+    grad = torch.empty((s.nx, s.ny), dtype=torch.float64, device=dev)
+    loss = C.c_double()
+    N.check(s._lib.wust_oracle_eval(s._handle(), tx.ctypes.data_as(C.c_void_p), tx.size,
+                                    rx.ctypes.data_as(C.c_void_p), rx.size, C.c_void_p(Yd.data_ptr()),
+                                    C.c_void_p(Md.data_ptr()), batch, 1, int(retain), TOL, 30, C.byref(loss),
+                                    C.c_void_p(grad.data_ptr()), G._stream(dev)))
+    torch.cuda.synchronize()
+    return loss.value, grad.t().cpu().numpy(), Md
+
+
+def _curv(s, direction, Md):
+    d = G._grid_to_dev(direction, s.device)
+    kappa = C.c_double()
+    N.check(s._lib.wust_directional_curvature(s._handle(), C.c_void_p(d.data_ptr()), C.c_void_p(Md.data_ptr()),
+                                              TOL, 30, C.byref(kappa), G._stream(s.device)))
+    return kappa.value
+
+
+def _close(a, b, tol):
+    return abs(a - b) <= tol * abs(b)
+
+
+def test_solve_then_growing_batches(case, obs):
+    """A dense 40-RHS solve fills the right-hand-side buffer; evaluations with batch 16 and then 40 on the
+    same handle must match fresh handles (the zero-kept region is tracked by size, not by a flag)."""
+    s = _solver(case)
+    rng = np.random.default_rng(3)
+    b = np.zeros((case.ny, case.nx, 40), np.complex128)
+    b[1:-1, 1:-1] = rng.standard_normal((case.ny - 2, case.nx - 2, 40))
+    s.solve(b, False)
+    allt = np.arange(case.n_tx)
+    l16, g16, _ = _eval(s, obs, allt, allt, 16)
+    l40, g40, _ = _eval(s, obs, allt, allt, 40)
+    lf16, gf16, _ = _eval(_solver(case), obs, allt, allt, 16)
+    lf40, gf40, _ = _eval(_solver(case), obs, allt, allt, 40)
+    assert _close(l16, lf16, 1e-10) and _close(l40, lf40, 1e-10), (l16, lf16, l40, lf40)
+    assert np.linalg.norm(g16 - gf16) <= 1e-9 * np.linalg.norm(gf16)
+    assert np.linalg.norm(g40 - gf40) <= 1e-9 * np.linalg.norm(gf40)
+    # batching does not change the objective (Runtime/tests/test_runtime.m:57-61)
+    assert _close(l16, l40, 1e-10)
+
+
+def test_curvature_after_shrinking_evaluations(case, obs):
+    """Curvature after an evaluation with 40 TX / 40 RX and then a retaining one with 8 TX / 12 RX reads the
+    receivers of the second, as a fresh handle that only ran the second does."""
+    rng = np.random.default_rng(4)
+    direction = rng.standard_normal((case.ny, case.nx))
+    allt = np.arange(case.n_tx)
+    tx8, rx12 = allt[::5], allt[1::3][:12]
+    s = _solver(case)
+    _eval(s, obs, allt, allt, 40)
+    l, _, Md = _eval(s, obs, tx8, rx12, 8, retain=True)
+    k = _curv(s, direction, Md)
+    f = _solver(case)
+    lf, _, Mf = _eval(f, obs, tx8, rx12, 8, retain=True)
+    kf = _curv(f, direction, Mf)
+    assert _close(l, lf, 1e-10)
+    assert kf > 0 and _close(k, kf, 1e-9), (k, kf)

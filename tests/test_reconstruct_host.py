@@ -91,3 +91,26 @@ def test_receiver_counts_and_boundary_sampling(runtime_case, runtime_obs):
     boundary = dict(runtime_obs, rx_index=np.r_[row + 1, rx[1:]], mask=mask)
     R.prepare_reconstruction(boundary, init, dict(one, max_updates=1))
     _reject(boundary, init, dict(one, evaluate_final_residual=True))
+
+
+def test_directional_curvature_rejects_other_receivers():
+    """directional_curvature checks rx_index against the receivers of the retained state before any
+    device work (WUST:InvalidInput), and accepts the same set in another integer type."""
+    import pytest as _pt
+
+    from waveforminversionust_amd import helmholtz as G
+    from waveforminversionust_amd._native import WustError
+
+    class _NoSolver:
+        def __getattr__(self, name):
+            raise AssertionError("the check must come before any device work")
+
+    state = G.OracleState(solver=_NoSolver(), scales=np.zeros(2, complex), forward_rhs=2, adjoint_rhs=2,
+                          extra=dict(rx_index=np.array([5, 9, 13], np.int64)))
+    with _pt.raises(WustError) as ei:
+        G.directional_curvature(state, np.zeros((4, 4)), [5, 9, 14])
+    assert ei.value.identifier == "WUST:InvalidInput"
+    with _pt.raises(WustError):
+        G.directional_curvature(state, np.zeros((4, 4)), [5, 9])
+    with _pt.raises(AssertionError):   # same receivers (int32): passes the check, reaches the solver
+        G.directional_curvature(state, np.zeros((4, 4)), np.array([5, 9, 13], np.int32))

@@ -51,6 +51,7 @@ class Info(C.Structure):
         ("assembly_ms", C.c_double), ("factor_ms", C.c_double),
         ("min_pivot_ratio", C.c_double), ("factor_bytes", C.c_int64),
         ("inverse_residual", C.c_double), ("iterative_blocks", C.c_int32), ("factor_fallback", C.c_int32),
+        ("gj_residual", C.c_double),
     ]
 
 

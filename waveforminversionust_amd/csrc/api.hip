@@ -5,6 +5,7 @@
 #include <map>
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -178,7 +179,9 @@ struct wust_helmholtz {
     PlainLib plain;
     int ws_Rpad = 0;
     cd *b64 = nullptr, *t64 = nullptr, *u64 = nullptr, *lam64 = nullptr;
-    bool b64_zero = false;      // b64 holds only zeros (the oracle evaluation undoes what it scatters)
+    // leading elements of b64 known to hold zeros (the oracle evaluation undoes what it scatters); the
+    // solve layout of a batch of Rpad right-hand sides is the first field_elems(Rpad) elements
+    int64_t b64_zero_elems = 0;
     cf *s32 = nullptr, *d32 = nullptr, *d32b = nullptr, *b32 = nullptr;
     uint8_t *bmask = nullptr;   // per 64 values of the current right-hand side: any nonzero
     bool bmask_valid = false;
@@ -188,6 +191,7 @@ struct wust_helmholtz {
     cd *scale = nullptr;        // [Rpad]
     int *idx = nullptr;         // scratch for node coordinates: 4 * max(ntx, nrx)
     int idx_cap = 0;
+    int idx_stride = 0;         // stride between the four coordinate arrays of the last evaluation
     std::vector<Retained> retained;
     std::vector<int> rx_x, rx_y;
     int ret_ntx = 0, ret_nrx = 0;
@@ -204,7 +208,7 @@ struct wust_helmholtz {
             dpool(op.w64, 9 * (int64_t)op.m * op.n);
             dpool(op.w32, 9 * (int64_t)op.m * op.npad);
             dpool(op.pages, pg * op.m);
-            dpool(op.work, 4 * pg);
+            dpool(op.work, 6 * pg);
             if (op.nswork) dpool(op.nswork, 6 * pg);
         }
         release_ws();
@@ -219,6 +223,7 @@ struct wust_helmholtz {
         dfree(op.ns_norm);
         dfree(op.ns_rounds);
         dfree(op.colflags);
+        dfree(op.gj_check);
         dfree(op.status);
         dfree(idx);
     }
@@ -267,7 +272,7 @@ struct wust_helmholtz {
         release_ws();
         int64_t cnt = field_elems(Rpad);
         b64 = dalloc<cd>(cnt);
-        b64_zero = false;
+        b64_zero_elems = 0;
         t64 = dalloc<cd>(cnt);
         u64 = dalloc<cd>(cnt);
         lam64 = dalloc<cd>(cnt);
@@ -365,6 +370,16 @@ struct wust_helmholtz {
             for (int r = 0; r < R; ++r)
                 if (h[r] > 0) worst = std::max(worst, std::sqrt(h[Rpad + r] / h[r]));
                 else if (h[Rpad + r] > 0) worst = INFINITY;
+            if (const char *e = std::getenv("WUST_REFINE_LOG"); e && e[0] == '1') {
+                std::vector<double> rel;
+                for (int r = 0; r < R; ++r) rel.push_back(h[r] > 0 ? std::sqrt(h[Rpad + r] / h[r]) : 0.0);
+                std::sort(rel.begin(), rel.end());
+                int over = 0;
+                for (double v : rel) over += v > tol;
+                std::fprintf(stderr, "refine %s R=%d: min %.3g p50 %.3g p90 %.3g max %.3g, %d above tol %.3g\n",
+                             adjoint ? "adjoint" : "forward", R, rel.front(), rel[rel.size() / 2],
+                             rel[(rel.size() * 9) / 10], rel.back(), over, tol);
+            }
             return worst;
         };
         if (op.mode == WUST_MODE_PLAIN) {
@@ -533,14 +548,17 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         op.status = dalloc<uint32_t>(8);
         const int64_t pg = (int64_t)op.npad * op.npad;
         op.pages = dalloc<cf>(pg * op.m);
-        op.work = dalloc<cf>(4 * pg);
+        op.work = dalloc<cf>(6 * pg);
         op.pbuf = dalloc<cf>(4 * 32 * 32);
         if (mode != WUST_MODE_PLAIN && op.factor_method == FACTOR_AUTO) {
             op.nswork = dalloc<cf>(6 * pg);
             op.ns_norm = dalloc<float>(2 * 8);
             op.ns_rounds = dalloc<int>(op.m);
         }
-        if (mode != WUST_MODE_PLAIN) op.colflags = dalloc<int>(op.m);
+        if (mode != WUST_MODE_PLAIN) {
+            op.colflags = dalloc<int>(op.m);
+            op.gj_check = dalloc<float>(op.m);
+        }
         op.factor_bytes = (int64_t)sizeof(cf) * pg * op.m;
         WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, 8 * sizeof(uint32_t), s));
 
@@ -557,7 +575,7 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         uint32_t st = h->read_status(s);
         if (st & ST_NONFINITE) throw Error(WUST_NUMERICAL_FAILURE, "LDR-9 produced invalid wavenumber or derivative");
         if (mode == WUST_MODE_PLAIN) factor_plain(op, h->plain, s);
-        else factor_engine(op, s);
+        else factor_engine(op, h->plain, s);
         WUST_HIP_CHECK(hipEventRecord(e2, s));
         WUST_HIP_CHECK(hipEventSynchronize(e2));
         float ms1 = 0, ms2 = 0;
@@ -568,9 +586,6 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipEventDestroy(e2);
-        st = h->read_status(s);
-        if (st & (ST_ZERO_PIVOT | ST_NONFINITE_PIVOT))
-            throw Error(WUST_SINGULAR_SCHUR_BLOCK, "zero or non-finite pivot in a Schur block");
         *out = h.release();
     });
 }
@@ -614,6 +629,7 @@ int wust_helmholtz_info(const wust_helmholtz *h, wust_info *info) {
         info->inverse_residual = op.inverse_residual;
         info->iterative_blocks = op.ns_blocks;
         info->factor_fallback = op.factor_fallback;
+        info->gj_residual = op.gj_residual;
     });
 }
 
@@ -640,7 +656,7 @@ int wust_helmholtz_solve(wust_helmholtz *h, const void *d_src, void *d_out, int6
         WUST_HIP_CHECK(hipEventRecord(e0, s));
         const int Rpad = (int)round_up(R, 16);
         h->ensure_ws(Rpad);
-        h->b64_zero = false;
+        h->b64_zero_elems = 0;
         ext_to_solve64((const cd *)d_src, h->b64, h->op.ny, h->op.nx, h->op.npad, (int)R, Rpad, nullptr, s);
         wust_solve_stats st =
             h->solve_internal(h->b64, h->u64, (int)R, Rpad, adjoint != 0, tol, max_iter, stats != nullptr, s);
@@ -695,6 +711,7 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
         h->ret_nrx = (int)n_rx;
         const int cap = (int)std::max(n_tx, n_rx);
         h->ensure_idx(cap);
+        h->idx_stride = cap;
         int *rxx = h->idx, *rxy = h->idx + cap, *txx = h->idx + 2 * cap, *txy = h->idx + 3 * cap;
         WUST_HIP_CHECK(hipMemcpyAsync(rxx, h->rx_x.data(), sizeof(int) * n_rx, hipMemcpyHostToDevice, s));
         WUST_HIP_CHECK(hipMemcpyAsync(rxy, h->rx_y.data(), sizeof(int) * n_rx, hipMemcpyHostToDevice, s));
@@ -710,8 +727,12 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
             const int64_t cnt = h->field_elems(Rpad);
             // unit impulses at the batch's transmitters (Runtime/matlab/wust_oracle.m:31-34), into b64
             // kept zero between evaluations: each scatter below is undone after its solve
-            if (!h->b64_zero) WUST_HIP_CHECK(hipMemsetAsync(h->b64, 0, sizeof(cd) * cnt, s));
-            h->b64_zero = false;
+            if (h->b64_zero_elems < cnt) {
+                WUST_HIP_CHECK(hipMemsetAsync(h->b64, 0, sizeof(cd) * cnt, s));
+                h->b64_zero_elems = cnt;
+            }
+            const int64_t zero_after = h->b64_zero_elems;   // restored once this batch undoes its scatters
+            h->b64_zero_elems = 0;
             launch_impulses(h->b64, txx + first, txy + first, nb, op.npad, Rpad, s);
             cd *ufwd = h->u64;
             if (retain) ufwd = dalloc<cd>(cnt);
@@ -769,7 +790,7 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
                 }
                 // undo the adjoint source (written at the receivers' nodes for RHS r < nb)
                 launch_clear_points(adj, rxx, rxy, (int)n_rx, nb, op.npad, Rpad, s);
-                h->b64_zero = true;
+                h->b64_zero_elems = zero_after;
                 if (retain) {
                     Retained rt;
                     rt.u = ufwd;
@@ -807,7 +828,7 @@ int wust_directional_curvature(wust_helmholtz *h, const double *d_direction, con
         require(!h->retained.empty(), "no retained oracle state (call wust_oracle_eval with retain)");
         hipStream_t s = (hipStream_t)stream;
         Operator &op = h->op;
-        const int cap = h->idx_cap;
+        const int cap = h->idx_stride;
         int *rxx = h->idx, *rxy = h->idx + cap;
         double total = 0;
         for (auto &rt : h->retained) {
@@ -816,7 +837,7 @@ int wust_directional_curvature(wust_helmholtz *h, const double *d_direction, con
             ca.u = rt.u;
             ca.vsm = op.vsm;
             ca.direction = d_direction;
-            h->b64_zero = false;
+            h->b64_zero_elems = 0;
             ca.src = h->b64;
             ca.d = op.d;
             ca.e = op.e;

@@ -23,6 +23,12 @@
 // The k loop keeps DEPTH k-blocks of operands in flight (DEPTH + 1 register
 // sets rotated by unrolling, no register copies), so a wave waits only for
 // data requested DEPTH k-blocks earlier.
+//
+// FOLD = true (four-real-product form only): the MFMA accumulators hold one
+// k-block (16 products per element) at a time and are added into FP64 running
+// sums after every k-block.  A long FP32 accumulation over the whole reduction
+// loses accuracy in proportion to its length; the block solves' contraction in
+// iterative refinement is set by exactly that rounding, so the sweeps fold.
 #pragma once
 #include "wust_common.h"
 
@@ -33,19 +39,46 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 enum { A_COL = 0, A_CONJT = 1 };
 enum { B_KCOL = 0, B_ROW = 1 };
 
-template <int RB, int CB, int ALAY, int BLAY, bool M3, int DEPTH = 2>
+template <int RB, int CB, int ALAY, int BLAY, bool M3, int DEPTH = 2, bool FOLD = false>
 struct CgemmWave {
+    static_assert(!(FOLD && M3), "the FP64 fold serves the four-real-product form");
     // operands of one k-block: a[rb][s], b[cb][s]
     struct Ops {
         cf a[RB][4], b[CB][4];
     };
     v4f p1[RB][CB], p2[RB][CB], p3[RB][CB];   // p2 unused without M3
+    double q1[FOLD ? RB : 1][FOLD ? CB : 1][4], q3[FOLD ? RB : 1][FOLD ? CB : 1][4];   // FP64 sums (FOLD)
 
     __device__ __forceinline__ void zero() {
 #pragma unroll
         for (int x = 0; x < RB; ++x)
 #pragma unroll
             for (int y = 0; y < CB; ++y) p1[x][y] = p2[x][y] = p3[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
+        if constexpr (FOLD) {
+#pragma unroll
+            for (int x = 0; x < RB; ++x)
+#pragma unroll
+                for (int y = 0; y < CB; ++y)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) q1[x][y][r] = q3[x][y][r] = 0.0;
+        }
+    }
+
+    // FOLD: add the k-block held by the MFMA accumulators into the FP64 sums and clear them
+    __device__ __forceinline__ void fold() {
+        if constexpr (FOLD) {
+#pragma unroll
+            for (int x = 0; x < RB; ++x)
+#pragma unroll
+                for (int y = 0; y < CB; ++y) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        q1[x][y][r] += (double)p1[x][y][r];
+                        q3[x][y][r] += (double)p3[x][y][r];
+                    }
+                    p1[x][y] = p3[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
+                }
+        }
     }
 
     // A points at A[i0][0] of the layout, B at B[0][j0].
@@ -142,7 +175,10 @@ struct CgemmWave {
             for (int u = 0; u <= DEPTH; ++u) {
                 load(o[(u + DEPTH) % (DEPTH + 1)], A, lda, B, ldb, k0(t + u + DEPTH), g, c);
                 __builtin_amdgcn_sched_barrier(0);
-                if (t + u < nk) mma(o[u]);
+                if (t + u < nk) {
+                    mma(o[u]);
+                    fold();
+                }
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -155,6 +191,9 @@ struct CgemmWave {
             if (M3) {
                 re[r] = p1[rb][cb][r] - p2[rb][cb][r];
                 im[r] = p3[rb][cb][r] - p1[rb][cb][r] - p2[rb][cb][r];
+            } else if (FOLD) {
+                re[r] = (float)q1[rb][cb][r];
+                im[r] = (float)q3[rb][cb][r];
             } else {
                 re[r] = p1[rb][cb][r];
                 im[r] = p3[rb][cb][r];

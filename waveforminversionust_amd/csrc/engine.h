@@ -27,26 +27,32 @@ struct Operator {
     cd *ck2 = nullptr, *vsm = nullptr, *w64 = nullptr;
     cf *w32 = nullptr;
     cf *pages = nullptr;
-    cf *work = nullptr;           // factor ping-pong buffers: [2 chains][2][npad*npad]
+    cf *work = nullptr;           // factor buffers: [2 chains][3][npad*npad] (Schur block T, two more)
     cf *pbuf = nullptr;           // diagonal-block inverses handed between panel launches: [2 chains][2][32*32]
     cf *nswork = nullptr;         // Newton-Schulz iterates and cubic operand: [2 chains][3][npad*npad]
     float *ns_norm = nullptr;     // ||E_r||_F^2 per chain and round: [2 chains][8]
     int *ns_rounds = nullptr;     // Newton-Schulz rounds used per block, + 64 if the last was forced: [m]
     int *colflags = nullptr;      // per block column: 1 when a right-hand side has a nonzero there: [m]
+    float *gj_check = nullptr;    // ||I - T_j X_j||_F^2 of each Gauss-Jordan block (0 elsewhere): [m]
     uint32_t *status = nullptr;   // device status words: [0] flags, [2..3] pivot ratios (float),
                                   // [4] largest Newton-Schulz residual bound (float)
     double assembly_ms = 0, factor_ms = 0, min_pivot_ratio = 1.0;
     double inverse_residual = 0;  // max over Newton-Schulz blocks of the bound on ||I - T_j X_j||_F
     int ns_blocks = 0;            // blocks inverted by Newton-Schulz
-    int factor_fallback = 0;      // 1: Newton-Schulz missed its bound and every block was redone by Gauss-Jordan
+    int factor_fallback = 0;      // 1: Newton-Schulz missed its bound and every block was redone by Gauss-Jordan;
+                                  // 2: a Gauss-Jordan block failed its check and every block was redone pivoted
+    double gj_residual = 0;       // largest ||I - T_j X_j||_F over the Gauss-Jordan blocks
     int64_t factor_bytes = 0;
 
     const cf *coef(int dir, int block) const { return w32 + ((int64_t)dir * m + block) * npad; }
     cf *page(int j) const { return pages + (int64_t)j * npad * npad; }
 };
 
+struct PlainLib;
+
 // ---- factor (twisted block-tridiagonal Schur recurrence + Gauss-Jordan inverses) ----
-void factor_engine(Operator &op, hipStream_t s);
+// lib: rocSOLVER handle for the pivoted refactorization when a pivot-free block fails its check
+void factor_engine(Operator &op, PlainLib &lib, hipStream_t s);
 // single-chain Schur block T = D_j - L_j X_{j-1} U_{j-1} (T = D_0 for j = 0), X = op.page
 void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s);
 

@@ -18,10 +18,15 @@
 //     launch.
 //     The Schur blocks of the nine-point PML operator are column-diagonally
 //     dominant (partial pivoting selects the diagonal at every step), so the
-//     elimination is pivot-free; a zero or non-finite pivot is reported as
-//     WUST_SINGULAR_SCHUR_BLOCK, and the smallest |pivot| / max|entry below
-//     it| inside each 16x16 pivot block is tracked so that a loss of
-//     dominance is visible.
+//     elimination is pivot-free.  That assumption is checked for every block:
+//     after its inversion, one MFMA product forms E = I - T_j X_j and its
+//     Frobenius norm; a zero or non-finite pivot, or ||E||_F above the accepted
+//     bound, redoes the whole factorization with pivoted inverses (rocSOLVER
+//     getrf + getrs per block, the reference's method), and a block that
+//     getrf finds exactly singular raises WUST_SINGULAR_SCHUR_BLOCK, as the
+//     reference does from getrf's info (Runtime/solver/decompBlockLU.cu:339-342,
+//     373-374).  The smallest |pivot| / max|entry below it| inside each 16x16
+//     pivot block is also reported.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -34,8 +39,11 @@
 #include <cstring>
 #include <utility>
 
+#include <rocsolver/rocsolver.h>
+
 #include "cgemm.h"
 #include "engine.h"
+#include "plain.h"
 #include "profile.h"
 
 namespace wust {
@@ -108,6 +116,17 @@ __global__ void __launch_bounds__(256) schur_kernel(SchurParams p) {
     if (tk.nterms >= 1) term(tk.t[0]);
     if (tk.nterms >= 2) term(tk.t[1]);
     tk.T[r + (int64_t)ld * c] = val;
+}
+
+// X = I over the whole padded page (the pivoted path's getrs then overwrites the n x n block; the
+// padding keeps the identity that every page carries)
+__global__ void __launch_bounds__(256) page_identity_kernel(cf *X, int ld) {
+    const int r = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
+    if (r < ld) X[r + (int64_t)ld * c] = (r == c) ? mk(1.f, 0.f) : mk(0.f, 0.f);
+}
+void launch_identity(cf *X, int ld, hipStream_t s) {
+    hipLaunchKernelGGL(page_identity_kernel, dim3((ld + 255) / 256, ld), dim3(256), 0, s, X, ld);
+    WUST_LAUNCH_CHECK();
 }
 
 struct GjTask {
@@ -504,7 +523,9 @@ __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
     __shared__ float wsum[4];
     const int ld = p.npad, i0 = ti * NB, j0 = tj * NB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-    CgemmWave<2, 2, A_COL, B_KCOL, M3> acc;
+    // the residual E = I - T X (four real products) folds each k-block into FP64 sums: its accuracy
+    // sets the accuracy of the inverse that the update X (I + E) produces
+    CgemmWave<2, 2, A_COL, B_KCOL, M3, 2, !M3> acc;
     acc.zero();
     acc.run(A + i0, ld, B + (int64_t)j0 * ld, ld, wave, ld / 16, 4, g, c);
 
@@ -563,7 +584,8 @@ __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
         if (!(bound <= p.accept)) atomicOr(p.status, (uint32_t)ST_INVERSE_UNCONVERGED);
         if (bound >= 0.f) atomicMax(reinterpret_cast<int *>(p.resid_max), __float_as_int(bound));
         // rounds this block used; + 64 when the budget ran out before ||E||^2 reached c^2
-        *tk.rounds = (p.r + 1) + (n > p.c2 ? 64 : 0);
+        // (a non-finite norm counts as ran out, so that a diverged block reaches Gauss-Jordan next time)
+        *tk.rounds = (p.r + 1) + (!(n <= p.c2) ? 64 : 0);
     }
 }
 
@@ -612,6 +634,10 @@ namespace {
 constexpr float NS_Q2 = 2e-6f;
 constexpr float NS_C2 = 1.5874e-4f;   // (2e-6)^(2/3)
 constexpr float NS_ACCEPT = 1e-4f;
+// Gauss-Jordan blocks (pivot-free) are accepted when ||I - T_j X_j||_F stays below this; the FP32
+// inverses of these blocks reach ~1e-5, and an elimination that loses column dominance leaves
+// errors orders of magnitude larger.  WUST_GJ_ACCEPT overrides it (tests force the pivoted path).
+constexpr float GJ_ACCEPT = 1e-2f;
 
 // Newton-Schulz round budgets learned per operator shape and frequency: the rounds each
 // chain step needed in the last factorization of that key (both chains' maximum; one more
@@ -619,8 +645,9 @@ constexpr float NS_ACCEPT = 1e-4f;
 // that budget in the interior columns, where one cubic round suffices with a wide margin,
 // and one spare round in the PML columns, whose convergence depends more on the model.  A
 // budget that still proves too small costs accuracy within the accepted bound or, beyond
-// it, the Gauss-Jordan fallback of that factorization, after which the key starts again
-// from the default budgets.
+// it, the Gauss-Jordan fallback of that factorization; the key then keeps that
+// factorization's profile, in which the steps that ran out of rounds need one more than
+// they had, so that they reach Gauss-Jordan in the next factorization.
 struct BudgetKey {
     int ny, nx, npad, dispersion;
     double f, dx, dy, a0, L;
@@ -642,6 +669,7 @@ struct FactorRun {
     hipStream_t s;
     bool allow_ns;
     int pml_cells;
+    PlainLib *pivoted = nullptr;    // non-null: every block by rocSOLVER getrf + getrs (partial pivoting)
     int ns_blocks = 0;
     int update_cap = NS_KMAX;
     std::vector<uint8_t> learned;   // per chain step; empty: the default budget
@@ -690,7 +718,7 @@ struct FactorRun {
             tk.Dc = op.coef(DC, j);
             tk.Ds = op.coef(DS, j);
             tk.nterms = 0;
-            tk.T = op.work + (int64_t)c * 2 * pg;
+            tk.T = op.work + (int64_t)c * 3 * pg;   // chain c's [T | ping | pong]
             tk.clear_norms = (allow_ns && op.ns_norm) ? op.ns_norm + c * NS_KMAX : nullptr;
             int kind = steps[c].kind;
             if ((kind == 0 || kind == 2) && j >= 1)  // - L_j X^L_{j-1} U_{j-1}
@@ -700,6 +728,24 @@ struct FactorRun {
         }
         dim3 grid((ld + 255) / 256, ld, sp.ntasks);
         launch_k(PC_SCHUR, schur_kernel, grid, dim3(256), 0, s, sp);
+
+        if (pivoted) {
+            // the reference's inversion: partial-pivot LU of T, then X = T^-1 by getrs against I
+            rocblas_handle hb = (rocblas_handle)pivoted->handle;
+            for (int c = 0; c < sp.ntasks; ++c) {
+                const int j = steps[c].block;
+                cf *T = sp.task[c].T;
+                cf *X = op.page(j);
+                if (rocsolver_cgetrf(hb, n, n, (rocblas_float_complex *)T, ld, pivoted->ipiv,
+                                     pivoted->info + 2 * (j % 65536)) != rocblas_status_success)
+                    throw Error(WUST_LIBRARY_ERROR, "rocsolver_cgetrf failed");
+                launch_identity(X, ld, s);
+                if (rocsolver_cgetrs(hb, rocblas_operation_none, n, n, (rocblas_float_complex *)T, ld, pivoted->ipiv,
+                                     (rocblas_float_complex *)X, ld) != rocblas_status_success)
+                    throw Error(WUST_LIBRARY_ERROR, "rocsolver_cgetrs failed");
+            }
+            return;
+        }
 
         if (!gj.empty()) {
             const int Q = ld / NB;
@@ -712,16 +758,35 @@ struct FactorRun {
                 gp.pivot_ratio = ratio_d;
                 for (int t = 0; t < gp.ntasks; ++t) {
                     const int c = gj[t];
-                    cf *b0 = op.work + (int64_t)c * 2 * pg;
-                    cf *b1 = b0 + pg;
-                    gp.task[t].src = (q % 2 == 0) ? b0 : b1;
-                    gp.task[t].dst = (q == Q - 1) ? op.page(steps[c].block) : ((q % 2 == 0) ? b1 : b0);
+                    // T stays intact for the residual check: the panels ping-pong between the
+                    // chain's two other buffers
+                    cf *T = op.work + (int64_t)c * 3 * pg, *ping = T + pg, *pong = T + 2 * pg;
+                    gp.task[t].src = (q == 0) ? T : ((q % 2 == 1) ? ping : pong);
+                    gp.task[t].dst = (q == Q - 1) ? op.page(steps[c].block) : ((q % 2 == 0) ? ping : pong);
                     cf *pb = op.pbuf + (int64_t)c * 2 * NB * NB;
                     gp.task[t].pin = (q == 0) ? nullptr : pb + (q % 2) * NB * NB;
                     gp.task[t].pout = (q + 1 < Q) ? pb + ((q + 1) % 2) * NB * NB : nullptr;
                 }
                 launch_k(PC_GJ_PANEL, gj_panel_kernel, dim3(gp.ntasks * Q * Q), dim3(256), 0, s, gp);
             }
+            // the pivot-free elimination's check: E = I - T X_j and ||E||_F^2 into the block's slot
+            NsParams cp{};
+            cp.ntasks = (int)gj.size();
+            cp.npad = ld;
+            cp.kind = NS_RESIDUAL;
+            cp.r = 0;
+            cp.R = 1;
+            for (int t = 0; t < cp.ntasks; ++t) {
+                const int c = gj[t];
+                NsTask &tk = cp.task[t];
+                cf *Tc = op.work + (int64_t)c * 3 * pg;
+                tk.T = Tc;
+                tk.X = op.page(steps[c].block);
+                tk.E = Tc + 2 * pg;
+                tk.norms = op.gj_check + steps[c].block;
+            }
+            const int Tt = ld / NB, G = cp.ntasks * Tt * Tt;
+            launch_k(PC_NS_GEMM, ns_gemm_kernel<false>, dim3(8 * ((G + 7) / 8)), dim3(256), 0, s, cp);
         }
         if (!ns.empty()) {
             const int R = ns_budget(st);
@@ -743,7 +808,7 @@ struct FactorRun {
                     const int c = ns[t];
                     const int j = steps[c].block;
                     NsTask &tk = np.task[t];
-                    cf *w = op.work + (int64_t)c * 2 * pg;       // [T | E]
+                    cf *w = op.work + (int64_t)c * 3 * pg;       // [T | E | -]
                     cf *xb = op.nswork + (int64_t)c * 3 * pg;    // [X_a | X_b | M]
                     tk.T = w;
                     tk.E = w + pg;
@@ -791,9 +856,10 @@ struct FactorRun {
 
 }  // namespace
 
-void factor_engine(Operator &op, hipStream_t s) {
+void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
     const uint32_t init[6] = {0u, 0u, 0x3f800000u, 0x3f800000u, 0u, 0u};   // flags, -, ratios 1.0f, residual 0
     WUST_HIP_CHECK(hipMemcpyAsync(op.status, init, sizeof(init), hipMemcpyHostToDevice, s));
+    WUST_HIP_CHECK(hipMemsetAsync(op.gj_check, 0, sizeof(float) * op.m, s));
     const int pml_cells = (int)std::ceil(op.ap.L / op.ap.dx);
     FactorRun run{op, s, op.factor_method == FACTOR_AUTO, pml_cells};
     // WUST_NS_MAX_UPDATES caps the Newton-Schulz rounds (one update each) per
@@ -860,6 +926,7 @@ void factor_engine(Operator &op, hipStream_t s) {
     if (st[0] & ST_INVERSE_UNCONVERGED) {
         // some block's Newton-Schulz iteration missed its bound: redo every block by Gauss-Jordan
         WUST_HIP_CHECK(hipMemcpyAsync(op.status, init, sizeof(init), hipMemcpyHostToDevice, s));
+        WUST_HIP_CHECK(hipMemsetAsync(op.gj_check, 0, sizeof(float) * op.m, s));
         FactorRun again{op, s, false, pml_cells};
         again.run();
         WUST_HIP_CHECK(hipMemcpyAsync(st, op.status, sizeof(st), hipMemcpyDeviceToHost, s));
@@ -872,6 +939,45 @@ void factor_engine(Operator &op, hipStream_t s) {
     op.min_pivot_ratio = r[0] < r[1] ? r[0] : r[1];
     op.inverse_residual = r[2];
     op.ns_blocks = run.ns_blocks;
+    // the pivot-free Gauss-Jordan blocks: ||I - T_j X_j||_F of each, and the zero / non-finite pivots
+    std::vector<float> gj((size_t)op.m);
+    WUST_HIP_CHECK(hipMemcpyAsync(gj.data(), op.gj_check, sizeof(float) * op.m, hipMemcpyDeviceToHost, s));
+    WUST_HIP_CHECK(hipStreamSynchronize(s));
+    float accept = GJ_ACCEPT;
+    if (const char *e = std::getenv("WUST_GJ_ACCEPT"); e && e[0]) accept = (float)std::atof(e);
+    double worst = 0;
+    bool bad = (st[0] & (ST_ZERO_PIVOT | ST_NONFINITE_PIVOT)) != 0;
+    for (float v : gj) {
+        if (!(v >= 0.f) || !std::isfinite(v)) bad = true;
+        else worst = std::max(worst, std::sqrt((double)v));
+    }
+    if (worst > accept) bad = true;
+    op.gj_residual = worst;
+    if (const char *e = std::getenv("WUST_NS_LOG"); e && e[0] == '1')
+        std::fprintf(stderr, "gauss-jordan check: %d x %d, largest ||I - T X||_F %.3g%s\n", op.ny, op.nx, worst,
+                     bad ? " -> pivoted refactorization" : "");
+    if (bad) {
+        // a pivot-free elimination lost accuracy (or met a zero pivot): every block again with partial
+        // pivoting; a block that getrf finds exactly singular is WUST_SINGULAR_SCHUR_BLOCK
+        lib.ensure(s, op.n);
+        const int ninfo = std::min(op.m, 65536);
+        WUST_HIP_CHECK(hipMemsetAsync(lib.info, 0, sizeof(int) * 2 * (size_t)ninfo, s));
+        WUST_HIP_CHECK(hipMemcpyAsync(op.status, init, sizeof(init), hipMemcpyHostToDevice, s));
+        FactorRun piv{op, s, false, pml_cells};
+        piv.pivoted = &lib;
+        piv.run();
+        std::vector<int> info(2 * (size_t)ninfo);
+        WUST_HIP_CHECK(hipMemcpyAsync(info.data(), lib.info, sizeof(int) * info.size(), hipMemcpyDeviceToHost, s));
+        WUST_HIP_CHECK(hipStreamSynchronize(s));
+        for (size_t i = 0; i < info.size(); i += 2)
+            if (info[i] > 0)
+                throw Error(WUST_SINGULAR_SCHUR_BLOCK, "getrf reported a singular Schur block (block " +
+                                                           std::to_string(i / 2) + ")");
+        op.factor_fallback = 2;
+        op.ns_blocks = 0;
+        op.min_pivot_ratio = 1.0;
+        op.inverse_residual = 0;
+    }
 }
 
 }  // namespace wust

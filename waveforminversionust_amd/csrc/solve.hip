@@ -26,9 +26,17 @@
 #include "engine.h"
 #include "profile.h"
 
+#ifndef WUST_SWEEP_FOLD
+#define WUST_SWEEP_FOLD 1
+#endif
+
 namespace wust {
 
 namespace {
+
+// The sweep products accumulate each k-block on MFMA and fold it into FP64 sums (CgemmWave FOLD): a
+// long FP32 accumulation over the block rows sets the refinement contraction of the whole solve.
+constexpr bool SWEEP_FOLD = WUST_SWEEP_FOLD != 0;
 
 struct SweepTerm {
     const cf *cn, *cc, *cs;   // tridiagonal row coefficients (block of the coupling)
@@ -107,7 +115,7 @@ __global__ void __launch_bounds__(256) sweep_gemm_kernel(SweepParams p) {
     const int i0 = mb * 16 * RB, j0 = nb * 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
 
-    CgemmWave<RB, 1, ALAY, B_ROW, false> acc;
+    CgemmWave<RB, 1, ALAY, B_ROW, false, 2, SWEEP_FOLD> acc;
     acc.zero();
     const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
     acc.run(A, npad, tk.B + j0, Rpad, wave, npad / 16, 4, g, c);
@@ -228,7 +236,7 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
             psrc[q] = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + (ng * TPW + q) * 16 + cc] : mk(0.f, 0.f);
     }
 
-    CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH> acc;
+    CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH, SWEEP_FOLD> acc;
     acc.zero();
     const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
     acc.run(A, npad, tk.B + j0, Rpad, ks, npad / 16, KS, g, c);
@@ -342,7 +350,7 @@ __device__ __forceinline__ void sweep_fused_body_tall(const FusedParams &p) {
         }
     }
 
-    CgemmWave<RB, 1, ALAY, B_ROW, false, DEPTH> acc;
+    CgemmWave<RB, 1, ALAY, B_ROW, false, DEPTH, SWEEP_FOLD> acc;
     acc.zero();
     const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
     acc.run(A, npad, tk.B + j0, Rpad, ks, npad / 16, KS, g, c);

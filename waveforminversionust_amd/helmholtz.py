@@ -30,7 +30,7 @@ import torch
 from . import _native as N
 from ._native import WustError
 
-DEFAULT_TOL = 3e-10
+DEFAULT_TOL = 1e-10
 DEFAULT_MAX_ITER = 30
 
 
@@ -302,16 +302,22 @@ def wust_oracle(c, obs, fi, cfg, nargout=2):
     N.check(solver._lib.wust_oracle_scales(solver._handle(), scales.ctypes.data_as(C.c_void_p)))
     state = OracleState(solver=solver, scales=scales, forward_rhs=tx.size,
                         adjoint_rhs=tx.size if need_gradient else 0, forward_seconds=elapsed,
-                        extra=dict(receiver_mask=Md))
+                        extra=dict(receiver_mask=Md, rx_index=rx))
     return loss.value, g, state
 
 
 def directional_curvature(state, direction, rx_index=None, receiver_mask=None):
     """kappa = sum_b || -(H^-1 M(v_b . delta))[rx] a_b . mask ||^2 with frozen a.
 
-    ``rx_index`` must be the one the state was built with (it is kept by the
-    engine); ``receiver_mask`` is ``obs.mask[:, :, fi]`` ([n_tx, n_rx]).
+    ``rx_index`` (may be None) must be the one the state was built with: the engine keeps the
+    receivers of the retained evaluation, so a different set raises ``WUST:InvalidInput``;
+    ``receiver_mask`` is ``obs.mask[:, :, fi]`` ([n_tx, n_rx]).
     """
+    if rx_index is not None:
+        kept = np.asarray(state.extra.get("rx_index", []), dtype=np.int64).ravel()
+        given = np.asarray(rx_index, dtype=np.int64).ravel()
+        if given.shape != kept.shape or not np.array_equal(given, kept):
+            raise WustError("WUST:InvalidInput", "rx_index differs from the receivers of the retained oracle state")
     solver = state.solver
     dev = solver.device
     d = _grid_to_dev(direction, dev)
