@@ -12,7 +12,10 @@ this size, so each check is one the mathematics fixes:
   transmitter impulses and receiver-supported right-hand sides, <= 5e-8 relative (both solves
   are accurate to ~2e-9 in the field);
 * gradient: central differences along two smooth directions with Richardson extrapolation,
-  <= 1e-5 relative (loss noise of the refined solves over the step).
+  <= 1e-5 relative (loss noise of the refined solves over the step);
+* against the FP64 oracle itself on 8 of the 256 transmitters (its sparse LU factors this grid in a
+  few seconds; the 256-transmitter solve would take minutes): refined misfit and gradient at the
+  1e-10 refinement tolerance <= 1e-8, and the source scales of a 256-transmitter evaluation.
 """
 import numpy as np
 import pytest
@@ -150,3 +153,38 @@ def test_metric_gradient_central_differences(metric):
         h = 1e-3
         rich = (4 * fd(h / 2) - fd(h)) / 3
         assert abs(rich - an) <= 1e-5 * abs(an), (rich, an)
+
+
+def test_metric_refined_matches_fp64_oracle():
+    """The north-star accuracy gate at the metric size and tolerance: refinement to a 1e-10 relative FP64
+    residual, gradient evaluated at 1500 m/s with observations from the true model.  On 8 of the 256
+    transmitters (all 256 receivers) the refined misfit and slowness gradient match the FP64 sparse-LU
+    oracle to <= 1e-8 (Runtime/matlab/wust_oracle.m:28-72); the evaluation of all 256 transmitters in one
+    batch fits the same source scales a* = p^H d / p^H p for those 8 (<= 1e-8)."""
+    case = W.metric_case()
+    f, ny, nx, ntx = case.frequencies_hz[0], case.ny, case.nx, case.n_tx
+    truth = G.HelmholtzSolver(case.x_m, case.y_m, case.c_true, None, f, -1, case.cfg["pml_strength"],
+                              case.cfg["pml_m"], dict(OPTS, tol=1e-10))
+    u, _ = truth.solve(_impulses(case, case.tx_index), False)
+    truth.close()
+    Y = u.permute(1, 0, 2).reshape(-1, ntx)[torch.as_tensor(case.rx_index - 1, device="cuda"), :].T.contiguous()
+    del u
+    sub = np.linspace(0, ntx - 1, 8).round().astype(int)
+    Yh = Y.cpu().numpy()
+    obs_sub = dict(x_m=case.x_m, y_m=case.y_m, shape_yx=(ny, nx), n_tx=len(sub), tx_index=case.tx_index[sub],
+                   rx_index=case.rx_index, frequencies_hz=[f], Y=Yh[sub][:, :, None],
+                   mask=np.ones((len(sub), case.n_rx, 1), bool))
+    cfg = dict(case.cfg, tol=1e-10)
+    c0h = np.full((ny, nx), 1500.0)
+    lo, go, so = H.wust_oracle(c0h, obs_sub, 0, dict(cfg, source_batch_size=len(sub)), 3)
+    c0 = torch.full((ny, nx), 1500.0, dtype=torch.float64, device="cuda")
+    lg, gg = G.wust_oracle(c0, obs_sub, 0, dict(cfg, source_batch_size=len(sub)))
+    gg = gg.cpu().numpy()
+    assert abs(lg - lo) <= 1e-8 * abs(lo), (lg, lo)
+    assert np.linalg.norm(gg - go) <= 1e-8 * np.linalg.norm(go), np.linalg.norm(gg - go) / np.linalg.norm(go)
+    obs_all = dict(obs_sub, n_tx=ntx, tx_index=case.tx_index, Y=Y[:, :, None],
+                   mask=torch.ones((ntx, case.n_rx, 1), dtype=torch.bool, device="cuda"))
+    _, _, state = G.wust_oracle(c0, obs_all, 0, dict(cfg, source_batch_size=ntx), nargout=3)
+    want = np.concatenate(so.scales)
+    got = state.scales[sub]
+    assert np.max(np.abs(got - want) / np.abs(want)) <= 1e-8, np.max(np.abs(got - want) / np.abs(want))
