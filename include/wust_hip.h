@@ -244,6 +244,18 @@ int wust_rf_dtft(const void *d_rf, int rf_dtype, int64_t n_time, int32_t n_rx, i
                  double dt, const double *frequencies_hz, int32_t n_freq, const uint8_t *d_active, void *d_pressure,
                  float *kernel_ms, void *stream);
 
+/* wust_rf_dtft without any device allocation: no reference counterpart.  For callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as wust_rf_dtft except kernel_ms (not offered); its work memory (the kernel
+ * table of one pass over the RF and a status word, 8 MiB) is a static array that each GPU holds until
+ * the process ends, so n_time is limited to WUST_DTFT_NOALLOC_MAX_TIME samples (WUST_INVALID_INPUT
+ * beyond it).  Every call on a GPU shares that GPU's array: calls that overlap on one GPU (other
+ * threads, other streams) race.  The call waits for its stream to read the status word. */
+#define WUST_DTFT_NOALLOC_MAX_TIME 16384
+int wust_rf_dtft_noalloc(const void *d_rf, int rf_dtype, int64_t n_time, int32_t n_rx, int32_t n_tx,
+                         const double *d_time_s, double dt, const double *frequencies_hz, int32_t n_freq,
+                         const uint8_t *d_active, void *d_pressure, void *stream);
+
 /* ---- tooling: per-kernel-class HIP-event timing (no reference counterpart) ------------ */
 /* Classes: 0 assembly, 1 Schur update, 2 Gauss-Jordan panel, 3 sweep operand,
  * 4 sweep GEMM, 5 FP64 residual, 6 source fit, 7 gradient, 8 plain factor step,

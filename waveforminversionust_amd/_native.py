@@ -99,6 +99,8 @@ EXPORTS = {
     "wust_linearized_step_length": (C.c_int, [C.c_double, C.c_double, C.POINTER(C.c_double)]),
     "wust_rf_dtft": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_double,
                                C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
+    "wust_rf_dtft_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
+                                       C.c_double, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "wust_project_slowness_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double,
                                                C.c_double, C.c_double, C.c_void_p, C.c_int64, C.c_void_p,
                                                C.POINTER(C.c_double), C.c_void_p]),

@@ -960,4 +960,20 @@ int wust_rf_dtft(const void *d_rf, int rf_dtype, int64_t n_time, int32_t n_rx, i
     });
 }
 
+int wust_rf_dtft_noalloc(const void *d_rf, int rf_dtype, int64_t n_time, int32_t n_rx, int32_t n_tx,
+                         const double *d_time_s, double dt, const double *frequencies_hz, int32_t n_freq,
+                         const uint8_t *d_active, void *d_pressure, void *stream) {
+    return guarded([&] {
+        require(d_rf && d_time_s && frequencies_hz && d_active && d_pressure, "null argument");
+        require(rf_dtype == 0 || rf_dtype == 1, "rf_dtype must be 0 (float32) or 1 (float64)");
+        require(n_time >= 2 && n_rx >= 1 && n_tx >= 1 && n_freq >= 1, "RF must be real [time,rx,tx] pressure");
+        require(std::isfinite(dt) && dt > 0, "Invalid sample times");
+        for (int i = 0; i < n_freq; ++i)
+            require(std::isfinite(frequencies_hz[i]) && frequencies_hz[i] > 0 && frequencies_hz[i] < 0.5 / dt,
+                    "Frequency outside saved Nyquist");
+        rf_dtft_noalloc(d_rf, rf_dtype == 1, n_time, n_rx, n_tx, d_time_s, dt, frequencies_hz, n_freq, d_active,
+                        d_pressure, (hipStream_t)stream);
+    });
+}
+
 }  // extern "C"

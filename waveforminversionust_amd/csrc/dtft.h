@@ -21,4 +21,11 @@ void rf_dtft(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx,
              const double *freqs_host, int n_freq, const uint8_t *active, void *out, hipStream_t s,
              float *kernel_ms);
 
+// The same DTFT with its work memory (status word, kernel table) in a static per-device array:
+// no device allocation.  n_time <= DTFT_NOALLOC_MAX_TIME, else WUST_INVALID_INPUT.
+constexpr int64_t DTFT_NOALLOC_MAX_TIME = WUST_DTFT_NOALLOC_MAX_TIME;
+void rf_dtft_noalloc(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx, const double *time_s,
+                     double dt, const double *freqs_host, int n_freq, const uint8_t *active, void *out,
+                     hipStream_t s);
+
 }  // namespace wust

@@ -22,6 +22,9 @@
 // balanced with FP64 MFMA for single input at ~12 frequencies); K is a small
 // L2-resident table staged through LDS one sample chunk at a time and shared
 // by the eight waves of a workgroup.
+#include <mutex>
+#include <string>
+
 #include "dtft.h"
 
 namespace wust {
@@ -34,12 +37,19 @@ constexpr int DT_WAVES = 8;        // waves per workgroup, 16 channels each
 constexpr int DT_KC = 64;          // samples per chunk (4 k-blocks of 16)
 constexpr int DT_LDS_PAD = 2;      // doubles of padding per LDS row
 
+// The (up to) 32 frequencies of one pass over the RF, passed by value.
+struct FreqChunk {
+    double f[32];
+};
+
 // K[row][t] for row tiles of 8 frequencies (re rows 0-7, im rows 8-15), zero past
 // n_time and past n_freq.  theta = (-2 pi f) t in that order of rounding, as the
-// reference's exp(-2i*pi*frequency_hz*time_s.') evaluates it.
-__global__ void dtft_table_kernel(const double *time_s, const double *freqs, int n_freq, int64_t n_time,
-                                  int64_t nt_pad, int rows, double dt, double *K) {
+// reference's exp(-2i*pi*frequency_hz*time_s.') evaluates it.  status (or null) is
+// cleared by the first thread, before the GEMM of the first pass reads it.
+__global__ void dtft_table_kernel(const double *time_s, FreqChunk freqs, int n_freq, int64_t n_time,
+                                  int64_t nt_pad, int rows, double dt, double *K, uint32_t *status) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (status && i == 0) *status = 0u;
     if (i >= (int64_t)rows * nt_pad) return;
     const int row = (int)(i / nt_pad);
     const int64_t t = i - (int64_t)row * nt_pad;
@@ -47,7 +57,7 @@ __global__ void dtft_table_kernel(const double *time_s, const double *freqs, int
     const int f = tile * 8 + (in & 7);
     double v = 0.0;
     if (f < n_freq && t < n_time) {
-        const double w = -2.0 * M_PI * freqs[f];
+        const double w = -2.0 * M_PI * freqs.f[f];
         const double th = w * time_s[t];
         double sn, cs;
         sincos(th, &sn, &cs);
@@ -197,17 +207,21 @@ struct Buf {
     }
 };
 
-}  // namespace
+// Largest kernel table of one pass: 4 row tiles (32 frequencies) of nt_pad doubles.
+int64_t table_elems(int64_t n_time) {
+    const int64_t nt_pad = (n_time + DT_KC - 1) / DT_KC * DT_KC;
+    return (int64_t)64 * nt_pad;
+}
 
-void rf_dtft(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx, const double *time_s, double dt,
-             const double *freqs_host, int n_freq, const uint8_t *active, void *out, hipStream_t s,
-             float *kernel_ms) {
+// The DTFT on caller-provided work memory: K (table_elems(n_time) doubles) and status (one word).
+// Passes over the RF run back to back on the stream (each pass's table kernel follows the
+// previous pass's GEMM in stream order, so one table buffer serves every pass); the status is
+// read once at the end.
+void dtft_run(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx, const double *time_s, double dt,
+              const double *freqs_host, int n_freq, const uint8_t *active, void *out, hipStream_t s,
+              float *kernel_ms, double *Kbuf, uint32_t *status) {
     const int64_t nt_pad = (n_time + DT_KC - 1) / DT_KC * DT_KC;
     const int64_t ncols = (int64_t)n_rx * n_tx;
-    Buf<double> freqs(n_freq);
-    Buf<uint32_t> status(1);
-    WUST_HIP_CHECK(hipMemcpyAsync(freqs.p, freqs_host, sizeof(double) * n_freq, hipMemcpyHostToDevice, s));
-    WUST_HIP_CHECK(hipMemsetAsync(status.p, 0, sizeof(uint32_t), s));
     const dim3 grid((unsigned)((ncols + 16 * DT_WAVES - 1) / (16 * DT_WAVES)));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (kernel_ms) {
@@ -220,17 +234,18 @@ void rf_dtft(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx,
         const int nf = n_freq - f0 < 32 ? n_freq - f0 : 32;
         const int rt = (nf + 7) / 8;
         const int rows = rt * 16;
-        Buf<double> K((int64_t)rows * nt_pad);
+        FreqChunk fc{};
+        for (int i = 0; i < nf; ++i) fc.f[i] = freqs_host[f0 + i];
         const int64_t nk = (int64_t)rows * nt_pad;
-        hipLaunchKernelGGL(dtft_table_kernel, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, time_s,
-                           freqs.p + f0, nf, n_time, nt_pad, rows, dt, K.p);
+        hipLaunchKernelGGL(dtft_table_kernel, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, time_s, fc,
+                           nf, n_time, nt_pad, rows, dt, Kbuf, f0 == 0 ? status : nullptr);
         WUST_LAUNCH_CHECK();
         DtftParams p{};
         p.rf = rf;
-        p.K = K.p;
+        p.K = Kbuf;
         p.active = active;
         p.out = out;
-        p.status = status.p;
+        p.status = status;
         p.n_time = n_time;
         p.nt_pad = nt_pad;
         p.ncols = ncols;
@@ -254,10 +269,9 @@ void rf_dtft(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx,
             WUST_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             total_ms += ms;
         }
-        WUST_HIP_CHECK(hipStreamSynchronize(s));   // K is freed at the end of the chunk
     }
     uint32_t st = 0;
-    WUST_HIP_CHECK(hipMemcpyAsync(&st, status.p, sizeof(st), hipMemcpyDeviceToHost, s));
+    WUST_HIP_CHECK(hipMemcpyAsync(&st, status, sizeof(st), hipMemcpyDeviceToHost, s));
     WUST_HIP_CHECK(hipStreamSynchronize(s));
     if (kernel_ms) {
         *kernel_ms = total_ms;
@@ -265,6 +279,55 @@ void rf_dtft(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx,
         (void)hipEventDestroy(e1);
     }
     if (st) throw Error(WUST_INVALID_INPUT, "Nonfinite RF on an active channel");
+}
+
+// Work memory of the allocation-free entry point: the status word (its own 256-byte line) and one
+// pass's kernel table, for n_time <= DTFT_NOALLOC_MAX_TIME.  One copy per GPU, held until the
+// process ends; every call on a GPU shares it, so calls that overlap on one GPU race.  Host code
+// never uses this name as an address: it resolves the device address per device once
+// (hipGetSymbolAddress) and passes that to the kernels.
+constexpr int64_t DTFT_NOALLOC_TABLE = (int64_t)64 * DTFT_NOALLOC_MAX_TIME;
+static __device__ __align__(256) double scratch[32 + DTFT_NOALLOC_TABLE];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64] = {};
+
+void *scratch_address() {
+    int dev = 0;
+    WUST_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw Error(WUST_RUNTIME_ERROR, "device index out of range for the DTFT scratch");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw Error(WUST_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+void rf_dtft(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx, const double *time_s, double dt,
+             const double *freqs_host, int n_freq, const uint8_t *active, void *out, hipStream_t s,
+             float *kernel_ms) {
+    Buf<uint32_t> status(1);
+    Buf<double> K(table_elems(n_time));
+    dtft_run(rf, is_double, n_time, n_rx, n_tx, time_s, dt, freqs_host, n_freq, active, out, s, kernel_ms, K.p,
+             status.p);
+}
+
+void rf_dtft_noalloc(const void *rf, bool is_double, int64_t n_time, int n_rx, int n_tx, const double *time_s,
+                     double dt, const double *freqs_host, int n_freq, const uint8_t *active, void *out,
+                     hipStream_t s) {
+    if (n_time > DTFT_NOALLOC_MAX_TIME)
+        throw Error(WUST_INVALID_INPUT, "n_time " + std::to_string(n_time) + " exceeds the allocation-free limit of " +
+                                            std::to_string(DTFT_NOALLOC_MAX_TIME) + " samples");
+    double *base = static_cast<double *>(scratch_address());
+    uint32_t *status = reinterpret_cast<uint32_t *>(base);
+    dtft_run(rf, is_double, n_time, n_rx, n_tx, time_s, dt, freqs_host, n_freq, active, out, s, nullptr, base + 32,
+             status);
 }
 
 }  // namespace wust
