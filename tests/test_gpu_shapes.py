@@ -166,7 +166,8 @@ def test_zero_column_skip_edges(monkeypatch, where):
 def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
     """Every kernel variant of the fused sweep step, forced in turn: the variants that split K over
     four waves (16-, 32- or 64-row tiles, any prefetch depth) accumulate each element in the same order
-    and give bit-identical solutions; those that split K over two waves agree to single-precision
+    and give bit-identical solutions; those that split K over two waves, and the split-K variant on
+    32 x 32 tiles (K over four workgroups, partials summed in FP64), agree to single-precision
     rounding.  Each solution solves the oracle's H to single-precision accuracy.  (52 rows: the
     taller tiles' last strip is clamped to the block end and the padding rows beyond the strips
     are zero.)"""
@@ -184,7 +185,7 @@ def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
     interior[1:-1, 1:-1] = True
     base = None
     for v, ksplit4 in [(0, True), (1, True), (2, False), (3, False), (4, True), (5, False), (6, True), (7, True),
-                      (8, True)]:
+                      (8, True), (9, False)]:
         monkeypatch.setenv("WUST_SWEEP_VARIANT", str(v))
         u = s.solve(b, adjoint)[0].clone()
         if base is None:

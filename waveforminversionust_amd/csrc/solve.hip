@@ -185,6 +185,8 @@ struct FusedParams {
     FusedTask task[2];
     int ntasks, n, npad, Rpad;
     int strips, units, per_xcd;
+    cf *skpart;               // split-K variant: partial tiles [tile][SK_PARTS][32][32]
+    int *skcount;             // split-K variant: arrivals per tile (a multiple of SK_PARTS between launches)
 };
 
 // KS waves share one 16 x 16 tile (k-blocks dealt round-robin, partial tiles summed in LDS),
@@ -415,6 +417,198 @@ __device__ __forceinline__ void sweep_fused_body_tall(const FusedParams &p) {
     }
 }
 
+// ---- split-K step on 32 x 32 tiles with operands shared through LDS ------------------------------
+//
+// The 16-row steps above are bound by operand traffic from L2: each wave streams its own A (16 rows of
+// the page) and B (16 RHS of the operand) over its K share, so every page row is read once per RHS
+// tile and every operand row once per strip (~170 MB per step at 576 x 256 against ~8 MB of distinct
+// data).  Here a workgroup's four waves compute the four 16 x 16 blocks of one 32-row (30 owned) x
+// 32-RHS tile over one quarter of K, with each k-block of A (32 rows) and B (32 RHS) staged once in
+// LDS and read by the two waves that share it: half the traffic per output element.  Splitting K
+// over SK_PARTS workgroups keeps the launch at ~4.75 waves per SIMD (1216 workgroups at 558 rows and
+// 256 RHS).  Each workgroup writes its FP32-rounded partial tile (the k-block fold's FP64 sums, as
+// the other variants round them before their LDS reduction); the last of a tile's workgroups to
+// arrive (an atomic count per tile) sums the SK_PARTS partials in part order in FP64, so the result
+// does not depend on the arrival order, and runs the fused epilogue.  The workgroups of one tile
+// are dealt to the same XCD (blockIdx % 8), whose L2 is their point of coherence: each waits for
+// its partial's stores to complete before counting itself in, and the last one invalidates its L1
+// before reading the others' partials.  No workgroup ever waits for another.
+constexpr int SK_PARTS = 4;
+constexpr int SK_TR = 32, SK_OWN = 30, SK_TC = 32;
+
+__device__ __forceinline__ void wait_vmem_stores() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#endif
+}
+__device__ __forceinline__ void invalidate_l1() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
+#endif
+}
+
+template <int ALAY>
+__device__ __forceinline__ void sk_load(const cf *X, const cf *B, int npad, int Rpad, int i0, int j0, int kb, int t,
+                                        cf (&a)[2], cf (&b)[2]) {
+    const int k0 = 16 * kb;
+    if (ALAY == A_COL) {   // A[i][k] = X[k npad + i]: thread t takes rows 2 (t % 16) + {0, 1} of k = t / 16
+        const cf *src = X + (int64_t)(k0 + (t >> 4)) * npad + i0 + 2 * (t & 15);
+        a[0] = src[0];
+        a[1] = src[1];
+    } else {               // A[i][k] = conj(X[i npad + k]): thread t takes k = 2 (t % 8) + {0, 1} of row t / 8
+        const float4 u = *reinterpret_cast<const float4 *>(X + (int64_t)(i0 + (t >> 3)) * npad + k0 + 2 * (t & 7));
+        a[0] = mk(u.x, -u.y);
+        a[1] = mk(u.z, -u.w);
+    }
+    const float4 v = *reinterpret_cast<const float4 *>(B + (int64_t)(k0 + (t >> 4)) * Rpad + j0 + 2 * (t & 15));
+    b[0] = mk(v.x, v.y);
+    b[1] = mk(v.z, v.w);
+}
+
+// LDS staging: As[k][row], Bs[k][col] of one k-block (k-major, so a lane's MFMA operand reads are
+// contiguous over its 16 rows / columns)
+template <int ALAY>
+__device__ __forceinline__ void sk_stage(cf (*As)[SK_TR], cf (*Bs)[SK_TC], int t, const cf (&a)[2], const cf (&b)[2]) {
+    if (ALAY == A_COL) {
+        *reinterpret_cast<float4 *>(&As[t >> 4][2 * (t & 15)]) = make_float4(a[0].re, a[0].im, a[1].re, a[1].im);
+    } else {
+        As[2 * (t & 7)][t >> 3] = a[0];
+        As[2 * (t & 7) + 1][t >> 3] = a[1];
+    }
+    *reinterpret_cast<float4 *>(&Bs[t >> 4][2 * (t & 15)]) = make_float4(b[0].re, b[0].im, b[1].re, b[1].im);
+}
+
+template <int ALAY>
+__global__ void __launch_bounds__(256) sweep_splitk_kernel(FusedParams p) {
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int tile = xcd * p.per_xcd + (local / SK_PARTS);
+    const int part = local % SK_PARTS;
+    if (tile >= p.units) return;
+    const int npad = p.npad, Rpad = p.Rpad;
+    const int Ng = Rpad / SK_TC;
+    const int ng = tile % Ng;
+    const int mb = (tile / Ng) % p.strips;
+    const int task = tile / (Ng * p.strips);
+    const FusedTask &tk = p.task[task];
+    const int o0 = SK_OWN * mb;
+    const int o1 = min(o0 + SK_OWN - 1, npad - 1);
+    const int i0 = min(max(o0 - 1, 0), npad - SK_TR);
+    const int j0 = ng * SK_TC;
+    const int nkb = npad / 16, per = (nkb + SK_PARTS - 1) / SK_PARTS;
+    const int kb0 = min(part * per, nkb), kb1 = min(kb0 + per, nkb);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, c = lane & 15;
+    const int rb = wave >> 1, cb = wave & 1;   // this wave's 16 x 16 block of the 32 x 32 tile
+
+    // one LDS block: the two k-block buffers of A and of B, later the 32 x 33 result tile
+    __shared__ cf sk_lds[2 * 16 * SK_TR + 2 * 16 * SK_TC];
+    __shared__ int last_flag;
+    cf(*As)[16][SK_TR] = reinterpret_cast<cf(*)[16][SK_TR]>(sk_lds);
+    cf(*Bs)[16][SK_TC] = reinterpret_cast<cf(*)[16][SK_TC]>(sk_lds + 2 * 16 * SK_TR);
+    v4f p1 = {0.f, 0.f, 0.f, 0.f}, p3 = p1;
+    double q1[4] = {0.0, 0.0, 0.0, 0.0}, q3[4] = {0.0, 0.0, 0.0, 0.0};
+    cf ra[2], rbv[2];
+    if (kb0 < kb1) {
+        sk_load<ALAY>(tk.X, tk.B, npad, Rpad, i0, j0, kb0, t, ra, rbv);
+        sk_stage<ALAY>(As[0], Bs[0], t, ra, rbv);
+    }
+    __syncthreads();
+    for (int kb = kb0; kb < kb1; ++kb) {
+        const int cur = (kb - kb0) & 1;
+        const bool more = kb + 1 < kb1;
+        if (more) sk_load<ALAY>(tk.X, tk.B, npad, Rpad, i0, j0, kb + 1, t, ra, rbv);
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const cf x = As[cur][4 * g + s4][16 * rb + c], y = Bs[cur][4 * g + s4][16 * cb + c];
+            p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.re, p1, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.im, p3, 0, 0, 0);
+            p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-x.im, y.im, p1, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.im, y.re, p3, 0, 0, 0);
+        }
+        // fold the k-block into the FP64 sums
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            q1[r] += (double)p1[r];
+            q3[r] += (double)p3[r];
+        }
+        p1 = v4f{0.f, 0.f, 0.f, 0.f};
+        p3 = p1;
+        if (more) sk_stage<ALAY>(As[cur ^ 1], Bs[cur ^ 1], t, ra, rbv);
+        __syncthreads();
+    }
+    // this workgroup's partial: lane holds rows 16 rb + 4 g + r, column 16 cb + c
+    cf *mine = p.skpart + ((int64_t)tile * SK_PARTS + part) * (SK_TR * SK_TC);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        mine[(16 * rb + 4 * g + r) * SK_TC + 16 * cb + c] = mk((float)q1[r], (float)q3[r]);
+    wait_vmem_stores();
+    __syncthreads();
+    if (t == 0) {
+        const int old = __hip_atomic_fetch_add(p.skcount + tile, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last_flag = (old % SK_PARTS) == SK_PARTS - 1;
+    }
+    __syncthreads();
+    if (!last_flag) return;
+    invalidate_l1();
+    // the tile: the partials summed in part order in FP64 (+ base), staged for the epilogue
+    cf(*tile_s)[SK_TC + 1] = reinterpret_cast<cf(*)[SK_TC + 1]>(sk_lds);   // 32 x 33 cf within the LDS block
+    const cf *parts = p.skpart + (int64_t)tile * SK_PARTS * (SK_TR * SK_TC);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = 16 * rb + 4 * g + r, col = 16 * cb + c;
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int q = 0; q < SK_PARTS; ++q) {
+            const cf v = parts[q * (SK_TR * SK_TC) + row * SK_TC + col];
+            sr += (double)v.re;
+            si += (double)v.im;
+        }
+        cf v = mk((float)sr, (float)si);
+        if (tk.base) v = v + tk.base[(int64_t)(i0 + row) * Rpad + j0 + col];
+        tile_s[row][col] = v;
+    }
+    __syncthreads();
+    // epilogue: thread t handles column t % 32 of rows t / 32 + 8 q
+    const int cc = t & 31;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int row = (t >> 5) + 8 * q, k = i0 + row;
+        if (k < o0 || k > o1) continue;
+        const int64_t off = (int64_t)k * Rpad + j0 + cc;
+        const cf v = tile_s[row][cc];
+        tk.out[off] = v;
+        if (!tk.Bn) continue;
+        cf pn = mk(0.f, 0.f), pc, ps = mk(0.f, 0.f);
+        if (!tk.transposed) {
+            pc = tk.cc[k];
+            if (k >= 1) pn = tk.cn[k];
+            if (k + 1 < npad) ps = tk.cs[k];
+        } else {
+            pc = conjf_(tk.cc[k]);
+            if (k >= 1) pn = conjf_(tk.cs[k - 1]);
+            if (k + 1 < npad) ps = conjf_(tk.cn[k + 1]);
+        }
+        const cf vn = (k >= 1) ? tile_s[row - 1][cc] : mk(0.f, 0.f);
+        const cf vs = (k + 1 < npad) ? tile_s[row + 1][cc] : mk(0.f, 0.f);
+        // the same products and order as operand_b
+        cf a = pc * v;
+        if (!tk.transposed) {
+            if (k >= 1) cfma(a, pn, vn);
+            if (k + 1 < npad) cfma(a, ps, vs);
+        } else {
+            if (k + 1 < npad) cfma(a, ps, vs);
+            if (k >= 1) cfma(a, pn, vn);
+        }
+        const cf src = tk.nsrc ? tk.nsrc[off] : mk(0.f, 0.f);
+        tk.Bn[off] = src - a;
+    }
+}
+
+// Clears the split-K arrival counts (a fill kernel, so the clear is an ordinary launch).
+__global__ void sk_clear_kernel(int *count, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) count[i] = 0;
+}
+
 // 1280 workgroups of four waves (40 strips x 16 RHS tiles x 2 chains at 558 rows and 256 RHS)
 // need five resident waves per SIMD to run in one round; with two k-blocks in flight the step fits
 // that register budget without spilling (deeper prefetch does not).  Launches that take several
@@ -449,19 +643,35 @@ __global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r
 // rows) = (2, 4, 16) five waves, (3, 4, 16), (4, 2, 16), (2, 2, 16) five waves, (2, 4, 16),
 // (2, 2, 16), (2, 4, 32), (3, 4, 32), (2, 4, 64); a variant with TPW tiles per workgroup needs
 // Rpad / 16 to be a multiple of TPW.
-constexpr int N_FUSED_VARIANTS = 9;
-constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2, 4, 2, 4, 4, 4};
-constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {2, 3, 4, 2, 2, 2, 2, 3, 2};
-constexpr bool FUSED_FIVE[N_FUSED_VARIANTS] = {true, false, false, true, false, false, false, false, false};
-constexpr int FUSED_TR[N_FUSED_VARIANTS] = {16, 16, 16, 16, 16, 16, 32, 32, 64};
+constexpr int N_FUSED_VARIANTS = 10;
+constexpr int SPLITK_VARIANT = 9;   // 32 x 32 tiles, K over SK_PARTS workgroups, operands through LDS
+constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2, 4, 2, 4, 4, 4, 4};
+constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {2, 3, 4, 2, 2, 2, 2, 3, 2, 1};
+constexpr bool FUSED_FIVE[N_FUSED_VARIANTS] = {true, false, false, true, false, false, false, false, false, false};
+constexpr int FUSED_TR[N_FUSED_VARIANTS] = {16, 16, 16, 16, 16, 16, 32, 32, 64, 32};
 
 // strips of variant v's tiles over n rows (each owns FUSED_TR - 2 rows), and the rows they cover
 inline int fused_strips(int v, int n) { return (n + FUSED_TR[v] - 3) / (FUSED_TR[v] - 2); }
 inline int fused_covered(int v, int n, int npad) { return std::min(fused_strips(v, n) * (FUSED_TR[v] - 2), npad); }
 
+// Workgroup geometry of variant v for fp's shape: strips, units (tiles; for the split-K variant one tile
+// per SK_PARTS workgroups) and units per XCD slot.
+void fused_geometry(int v, FusedParams &fp) {
+    fp.strips = fused_strips(v, fp.n);
+    if (v == SPLITK_VARIANT) {
+        fp.units = fp.ntasks * fp.strips * (fp.Rpad / SK_TC);
+        fp.per_xcd = (fp.units + 7) / 8;
+        return;
+    }
+    const int tpw = 4 / FUSED_KS[v];
+    fp.units = fp.ntasks * fp.strips * (fp.Rpad / 16) / tpw;
+    fp.per_xcd = (fp.units + 7) / 8;
+}
+inline int splitk_tiles(int n, int Rpad) { return 2 * ((n + SK_OWN - 1) / SK_OWN) * (Rpad / SK_TC); }
+
 template <int ALAY>
 void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool profiled) {
-    const dim3 grid(8 * fp.per_xcd), block(256);
+    const dim3 grid(8 * fp.per_xcd * (v == SPLITK_VARIANT ? SK_PARTS : 1)), block(256);
     auto go = [&](auto kernel) {
         if (profiled) launch_k(PC_SWEEP_GEMM, kernel, grid, block, 0, s, fp);
         else {
@@ -478,6 +688,7 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
         case 6: go(sweep_fused_kernel<ALAY, 2, 4, 32>); break;
         case 7: go(sweep_fused_kernel<ALAY, 3, 4, 32>); break;
         case 8: go(sweep_fused_kernel<ALAY, 2, 4, 64>); break;
+        case SPLITK_VARIANT: go(sweep_splitk_kernel<ALAY>); break;
         default: go(sweep_fused_kernel_d2<ALAY, 4>); break;
     }
 }
@@ -492,6 +703,7 @@ std::mutex g_fused_mu;
 std::map<FusedKey, int> g_fused_choice;
 
 bool fused_eligible(int v, const FusedParams &fp) {
+    if (v == SPLITK_VARIANT) return fp.skpart && fp.skcount && fp.Rpad % SK_TC == 0 && SK_TR <= fp.npad;
     return (fp.Rpad / 16) % (4 / FUSED_KS[v]) == 0 && FUSED_TR[v] <= fp.npad;
 }
 
@@ -528,13 +740,10 @@ int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
         else launch_fused_variant<A_COL>(v, q[v], s, false);
     };
     for (int v = 0; v < N_FUSED_VARIANTS; ++v) {
-        const int tpw = 4 / FUSED_KS[v];
         ok[v] = fused_eligible(v, fp);
         per_launch[v] = 1e30f;
         q[v] = fp;
-        q[v].strips = fused_strips(v, fp.n);
-        q[v].units = q[v].ntasks * q[v].strips * (q[v].Rpad / 16) / tpw;
-        q[v].per_xcd = (q[v].units + 7) / 8;
+        fused_geometry(v, q[v]);
         if (ok[v]) run(v);    // warms the code object and the caches
     }
     for (int round = 0; round < ROUNDS; ++round)
@@ -583,6 +792,12 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
     const int64_t blk = (int64_t)npad * Rpad;
     cf *yf = work;                                 // forward iterates y_j / z_j: [m][npad][Rpad]
     cf *ops = work + (int64_t)m * blk;             // operand blocks: [2 chains][2 ping-pong]
+    // split-K variant: partial tiles, then the arrival counts (cleared per solve)
+    cf *sk_part = work + ((int64_t)m + 4) * blk;
+    const int sk_tiles = splitk_tiles(op.n, Rpad);
+    int *sk_count = reinterpret_cast<int *>(sk_part + (int64_t)sk_tiles * SK_PARTS * SK_TR * SK_TC);
+    hipLaunchKernelGGL(sk_clear_kernel, dim3((sk_tiles + 255) / 256), dim3(256), 0, s, sk_count, sk_tiles);
+    WUST_LAUNCH_CHECK();
     if (unfused) {
         solve_engine_unfused(op, src, out, ops, Rpad, adjoint, s);
         return;
@@ -619,12 +834,11 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
         fp.n = op.n;
         fp.npad = npad;
         fp.Rpad = Rpad;
+        fp.skpart = sk_part;
+        fp.skcount = sk_count;
         fp.strips = fused_strips(0, op.n);
         if (variant < 0) variant = fused_variant(fp, adjoint, s);
-        const int tpw = 4 / FUSED_KS[variant];
-        fp.strips = fused_strips(variant, op.n);
-        fp.units = fp.ntasks * fp.strips * (Rpad / 16) / tpw;
-        fp.per_xcd = (fp.units + 7) / 8;
+        fused_geometry(variant, fp);
         prof_units(PC_SWEEP_GEMM, fp.ntasks);
         if (adjoint) launch_fused_variant<A_CONJT>(variant, fp, s, true);
         else launch_fused_variant<A_COL>(variant, fp, s, true);
