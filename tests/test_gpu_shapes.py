@@ -164,13 +164,11 @@ def test_zero_column_skip_edges(monkeypatch, where):
 @pytest.mark.parametrize("ny,nx", [(66, 40), (52, 37)])
 @pytest.mark.parametrize("adjoint", [False, True])
 def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
-    """Every kernel variant of the fused sweep step, forced in turn: the variants that split K over
-    four waves (16-, 32- or 64-row tiles, any prefetch depth) accumulate each element in the same order
-    and give bit-identical solutions; those that split K over two waves, and the split-K variant on
-    32 x 32 tiles (K over four workgroups, partials summed in FP64), agree to single-precision
-    rounding.  Each solution solves the oracle's H to single-precision accuracy.  (52 rows: the
-    taller tiles' last strip is clamped to the block end and the padding rows beyond the strips
-    are zero.)"""
+    """Every kernel variant of the fused sweep step, forced in turn: they differ only in how the
+    operands reach the MFMA (staged through LDS, buffer or flat loads, prefetch depth) and deal K
+    over the four waves in the same order, so they give bit-identical solutions, each solving the
+    oracle's H to single-precision accuracy.  (52 rows: the last strip is clamped to the block end
+    and the padding rows beyond the strips are zero.)"""
     x, y, c = _grid(ny, nx)
     f, a0, L = 2.2e5, 10.0, 2e-3
     R = 32
@@ -184,17 +182,13 @@ def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
     interior = np.zeros((ny, nx), bool)
     interior[1:-1, 1:-1] = True
     base = None
-    for v, ksplit4 in [(0, True), (1, True), (2, False), (3, False), (4, True), (5, False), (6, True), (7, True),
-                      (8, True), (9, False)]:
+    for v in range(5):
         monkeypatch.setenv("WUST_SWEEP_VARIANT", str(v))
         u = s.solve(b, adjoint)[0].clone()
         if base is None:
             base = u
-        elif ksplit4:
-            assert torch.equal(u, base), v
         else:
-            rel = float(torch.linalg.norm(u - base) / torch.linalg.norm(base))
-            assert rel <= 1e-5, (v, rel)
+            assert torch.equal(u, base), v
         res = (Hm @ u.cpu().numpy().reshape(-1, R, order="F") - bf).reshape(ny, nx, R, order="F")[interior]
         assert np.linalg.norm(res) <= 1e-4 * np.linalg.norm(bf), v
 

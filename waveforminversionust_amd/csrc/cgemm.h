@@ -184,6 +184,104 @@ struct CgemmWave {
         }
     }
 
+    // ---- the same k loop over buffer-resource loads ----
+    //
+    // Each operand is addressed as a raw buffer (base and byte extent in scalar registers).  A lane's
+    // part of an address is a 32-bit byte offset that stays fixed over the loop, the k-block's part a
+    // wave-uniform scalar offset, so the loop spends no vector instructions on addresses (the flat
+    // loads above need a 64-bit multiply-add per load), and a read past the operand's extent returns
+    // zero instead of touching memory.  kb0 and step must be wave-uniform.
+    struct BufOps {
+        __amdgpu_buffer_rsrc_t ra, rb;
+        uint32_t va[RB], vb[CB];        // lane byte offsets
+        uint32_t sa, sb;                // byte stride of one k for the layouts strided in k (A_COL, B_ROW)
+    };
+    __device__ __forceinline__ static BufOps buf_setup(const cf *A, int64_t lda, int64_t a_elems, const cf *B,
+                                                       int64_t ldb, int64_t b_elems, int g, int c) {
+        BufOps r;
+        r.ra = __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(A), (short)0, (int)(a_elems * 8), 0x00020000);
+        r.rb = __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(B), (short)0, (int)(b_elems * 8), 0x00020000);
+        r.sa = (uint32_t)(lda * 8);
+        r.sb = (uint32_t)(ldb * 8);
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            const uint32_t i = 16 * rb + c;
+            r.va[rb] = (ALAY == A_COL) ? (uint32_t)((4 * g * lda + i) * 8) : (uint32_t)((i * lda + 4 * g) * 8);
+        }
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+            const uint32_t j = 16 * cb + c;
+            r.vb[cb] = (BLAY == B_ROW) ? (uint32_t)((4 * g * ldb + j) * 8) : (uint32_t)((j * ldb + 4 * g) * 8);
+        }
+        return r;
+    }
+    // (the loaded vector is reinterpreted whole: this compiler narrows the load to one dword when the
+    // elements of the returned integer vector are extracted one by one)
+    typedef float v2f_ __attribute__((ext_vector_type(2)));
+    __device__ __forceinline__ static cf ld2(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        const v2f_ u = __builtin_bit_cast(v2f_, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+        return mk(u.x, u.y);
+    }
+    __device__ __forceinline__ static float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        const v4f u = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(r, v, s, 0));
+        return make_float4(u.x, u.y, u.z, u.w);
+    }
+    __device__ __forceinline__ static void load_buf(Ops &o, const BufOps &q, int k0) {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            if (ALAY == A_COL) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) o.a[rb][s] = ld2(q.ra, q.va[rb], (uint32_t)(k0 + s) * q.sa);
+            } else {
+                const float4 u0 = ld4(q.ra, q.va[rb], (uint32_t)k0 * 8), u1 = ld4(q.ra, q.va[rb], (uint32_t)k0 * 8 + 16);
+                o.a[rb][0] = mk(u0.x, -u0.y);
+                o.a[rb][1] = mk(u0.z, -u0.w);
+                o.a[rb][2] = mk(u1.x, -u1.y);
+                o.a[rb][3] = mk(u1.z, -u1.w);
+            }
+        }
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+            if (BLAY == B_KCOL) {
+                const float4 u0 = ld4(q.rb, q.vb[cb], (uint32_t)k0 * 8), u1 = ld4(q.rb, q.vb[cb], (uint32_t)k0 * 8 + 16);
+                o.b[cb][0] = mk(u0.x, u0.y);
+                o.b[cb][1] = mk(u0.z, u0.w);
+                o.b[cb][2] = mk(u1.x, u1.y);
+                o.b[cb][3] = mk(u1.z, u1.w);
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) o.b[cb][s] = ld2(q.rb, q.vb[cb], (uint32_t)(k0 + s) * q.sb);
+            }
+        }
+    }
+    // A / B as in run(); a_elems / b_elems are the complex elements readable from A / B.
+    __device__ __forceinline__ void run_buf(const cf *A, int64_t lda, int64_t a_elems, const cf *B, int64_t ldb,
+                                            int64_t b_elems, int kb0, int kb_end, int step, int g, int c) {
+        const int nk = (kb_end - kb0 + step - 1) / step;
+        if (nk <= 0) return;
+        const BufOps q = buf_setup(A, lda, a_elems, B, ldb, b_elems, g, c);
+        const int last = kb0 + (nk - 1) * step;
+        auto k0 = [&](int t) {
+            const int kb = kb0 + t * step;
+            return 16 * (kb < last ? kb : last);
+        };
+        Ops o[DEPTH + 1];
+#pragma unroll
+        for (int u = 0; u < DEPTH; ++u) load_buf(o[u], q, k0(u));
+        for (int t = 0; t < nk; t += DEPTH + 1) {
+#pragma unroll
+            for (int u = 0; u <= DEPTH; ++u) {
+                load_buf(o[(u + DEPTH) % (DEPTH + 1)], q, k0(t + u + DEPTH));
+                __builtin_amdgcn_sched_barrier(0);
+                if (t + u < nk) {
+                    mma(o[u]);
+                    fold();
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+
     // Complex result of block (rb, cb): lane holds rows 4g + r (r = 0..3), column c.
     __device__ __forceinline__ void result(int rb, int cb, float (&re)[4], float (&im)[4]) const {
 #pragma unroll

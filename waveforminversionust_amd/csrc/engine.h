@@ -65,10 +65,6 @@ void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s);
 // exactly zero); each chain still computes at least its last step.
 void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad, bool adjoint, hipStream_t s,
                   int lead = 0, int trail = 0);
-// + the split-K sweep variant's partial tiles (4 x 32 x 32 per 32 x 32 tile) and arrival counts
-inline int64_t solve_work_elems(const Operator &op, int Rpad) {
-    const int64_t tiles = 2 * (int64_t)((op.n + 29) / 30) * (Rpad / 32);
-    return ((int64_t)op.m + 4) * op.npad * Rpad + tiles * 4 * 32 * 32 + (tiles + 1) / 2;
-}
+inline int64_t solve_work_elems(const Operator &op, int Rpad) { return ((int64_t)op.m + 4) * op.npad * Rpad; }
 
 }  // namespace wust

@@ -522,12 +522,14 @@ __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
     __shared__ float4 red[4][4][2][64];   // [wave][block][half][lane]
     __shared__ float wsum[4];
     const int ld = p.npad, i0 = ti * NB, j0 = tj * NB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform (scalar k offsets)
     // the residual E = I - T X (four real products) folds each k-block into FP64 sums: its accuracy
     // sets the accuracy of the inverse that the update X (I + E) produces
     CgemmWave<2, 2, A_COL, B_KCOL, M3, 2, !M3> acc;
     acc.zero();
-    acc.run(A + i0, ld, B + (int64_t)j0 * ld, ld, wave, ld / 16, 4, g, c);
+    const int64_t nn = (int64_t)ld * ld;
+    acc.run_buf(A + i0, ld, nn - i0, B + (int64_t)j0 * ld, ld, nn - (int64_t)j0 * ld, wave, ld / 16, 4, g, c);
 
     // reduce the four k-partials: wave w finishes block (w / 2, w % 2)
 #pragma unroll

@@ -165,10 +165,9 @@ constexpr int SWEEP_RB = 1;   // 16-row tiles: 2 * 36 * 16 = 1152 WGs at 560^2 x
 // computes rows i0 .. i0 + 15 with i0 = clamp(14 mb - 1, 0, npad - 16), so every owned row
 // has both neighbours in the tile (or outside the block, where they are zero).  The owned
 // rows of v are stored and B' is formed from the tile in LDS; the separate operand pass and
-// its launch disappear, at the cost of 16/14 of the MFMA work (32-row tiles that own 30 rows
-// cost 32/30, with half the workgroups).  Back substitution reads its base (y_j) from a
-// different array than it writes (u_j), so halo rows are never read after a neighbour
-// rewrote them.
+// its launch disappear, at the cost of 16/14 of the MFMA work.  Back substitution reads its
+// base (y_j) from a different array than it writes (u_j), so halo rows are never read after a
+// neighbour rewrote them.
 constexpr int FUSED_OWN = 14;   // owned rows of a 16-row tile
 
 struct FusedTask {
@@ -185,16 +184,16 @@ struct FusedParams {
     FusedTask task[2];
     int ntasks, n, npad, Rpad;
     int strips, units, per_xcd;
-    cf *skpart;               // split-K variant: partial tiles [tile][SK_PARTS][32][32]
-    int *skcount;             // split-K variant: arrivals per tile (a multiple of SK_PARTS between launches)
 };
 
-// KS waves share one 16 x 16 tile (k-blocks dealt round-robin, partial tiles summed in LDS),
-// so a workgroup of four waves covers TPW = 4 / KS adjacent RHS tiles of one strip; DEPTH
-// k-blocks of operands are in flight per wave.  The variant is picked per shape at run time.
-template <int ALAY, int DEPTH, int KS>
+// The four waves of a workgroup share one 16 x 16 tile (k-blocks dealt round-robin, partial tiles
+// summed in LDS in wave order); DEPTH k-blocks of operands are in flight per wave.  BUF: buffer
+// loads (scalar k offsets) instead of flat loads; LATE: the epilogue operands are loaded after the
+// GEMM instead of before it (fewer registers live across the k loop).
+template <int ALAY, int DEPTH, bool LATE = false, bool BUF = false>
 __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
-    constexpr int TPW = 4 / KS;
+    constexpr int KS = 4, TPW = 1, NW = 4;
+    constexpr bool FOLD = SWEEP_FOLD;
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
     const int L = xcd * p.per_xcd + local;
     if (L >= p.units) return;
@@ -207,7 +206,8 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
     const int o0 = FUSED_OWN * mb;
     const int o1 = min(o0 + FUSED_OWN - 1, npad - 1);
     const int i0 = min(max(o0 - 1, 0), npad - 16);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform (scalar k offsets)
     const int tw = wave / KS, ks = wave % KS;
     const int j0 = (ng * TPW + tw) * 16;
 
@@ -218,32 +218,41 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
     const int k = i0 + t;
     const bool own = k >= o0 && k <= o1;
     cf pbase[4];
-    if (tk.base && ks == 0) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) pbase[r] = tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
-    }
     cf pn = mk(0.f, 0.f), pc = mk(0.f, 0.f), ps = mk(0.f, 0.f), psrc[TPW];
-    if (own && tk.Bn) {
-        if (!tk.transposed) {
-            pc = tk.cc[k];
-            if (k >= 1) pn = tk.cn[k];
-            if (k + 1 < npad) ps = tk.cs[k];
-        } else {
-            pc = conjf_(tk.cc[k]);
-            if (k >= 1) pn = conjf_(tk.cs[k - 1]);
-            if (k + 1 < npad) ps = conjf_(tk.cn[k + 1]);
-        }
+    auto load_epilogue = [&] {
+        if (tk.base && ks == 0) {
 #pragma unroll
-        for (int q = 0; q < TPW; ++q)
-            psrc[q] = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + (ng * TPW + q) * 16 + cc] : mk(0.f, 0.f);
-    }
+            for (int r = 0; r < 4; ++r) pbase[r] = tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
+        }
+        if (own && tk.Bn) {
+            if (!tk.transposed) {
+                pc = tk.cc[k];
+                if (k >= 1) pn = tk.cn[k];
+                if (k + 1 < npad) ps = tk.cs[k];
+            } else {
+                pc = conjf_(tk.cc[k]);
+                if (k >= 1) pn = conjf_(tk.cs[k - 1]);
+                if (k + 1 < npad) ps = conjf_(tk.cn[k + 1]);
+            }
+#pragma unroll
+            for (int q = 0; q < TPW; ++q)
+                psrc[q] = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + (ng * TPW + q) * 16 + cc] : mk(0.f, 0.f);
+        }
+    };
+    if (!LATE) load_epilogue();
 
-    CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH, SWEEP_FOLD> acc;
+    CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH, FOLD> acc;
     acc.zero();
-    const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
-    acc.run(A, npad, tk.B + j0, Rpad, ks, npad / 16, KS, g, c);
+    const int64_t a_off = (ALAY == A_COL) ? i0 : (int64_t)i0 * npad;
+    const cf *A = tk.X + a_off;
+    if (BUF)
+        acc.run_buf(A, npad, (int64_t)npad * npad - a_off, tk.B + j0, Rpad, (int64_t)npad * Rpad - j0, ks, npad / 16,
+                    KS, g, c);
+    else
+        acc.run(A, npad, tk.B + j0, Rpad, ks, npad / 16, KS, g, c);
+    if (LATE) load_epilogue();
 
-    __shared__ float4 red[4][2][64];
+    __shared__ float4 red[NW][2][64];
     __shared__ cf tile[TPW][16][17];
     float re[4], im[4];
     acc.result(0, 0, re, im);
@@ -294,290 +303,121 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
     }
 }
 
-// The same step on TR-row tiles (TR a multiple of 16; TR - 2 rows owned, a wave accumulating
-// TR / 16 row blocks so that each operand block it loads serves all of them).  Kept apart from
-// the 16-row body so that each compiles to its own register allocation.
-template <int ALAY, int DEPTH, int KS, int TR>
-__device__ __forceinline__ void sweep_fused_body_tall(const FusedParams &p) {
-    constexpr int TPW = 4 / KS, RB = TR / 16, OWN = TR - 2;
-    constexpr int EPT = RB * TPW;   // epilogue elements per thread
+// The 16-row step with operands loaded 16 bytes per lane.  The MFMA wants lane (g, c) to hold row
+// c of A (and column c of B) at k = 4 g + s, which no layout here stores contiguously per lane, so
+// the flat loads above move 8 bytes per lane per instruction.  The per-CU load path costs about the
+// same per instruction for 8 and 16 bytes per lane when a wave's addresses cover whole cache lines,
+// so here each wave fetches its k-block (16 x 16 of A and of B, 2 KiB each) as four 16-byte-per-lane
+// buffer loads of whole lines into registers, writes them to a wave-private LDS tile in [k][row]
+// order (transposing A for the adjoint), and reads the MFMA operands back from LDS.  Each wave uses
+// only its own LDS tile and LDS operations of one wave complete in order, so no barrier is needed;
+// DEPTH k-blocks of loads are in flight.  Same K split, accumulation order and epilogue as the
+// 16-row body with KS = 4, so the results are bit-identical to it.
+template <int ALAY, int DEPTH>
+__device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
+    constexpr int KS = 4, STR = 17;   // LDS row stride in complex elements (16 + 1 pad: no bank conflicts)
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
     const int L = xcd * p.per_xcd + local;
     if (L >= p.units) return;
     const int npad = p.npad, Rpad = p.Rpad;
-    const int Ng = Rpad / 16 / TPW;
+    const int Ng = Rpad / 16;
     const int ng = L % Ng;
     const int mb = (L / Ng) % p.strips;
     const int task = L / (Ng * p.strips);
     const FusedTask &tk = p.task[task];
-    const int o0 = OWN * mb;
-    const int o1 = min(o0 + OWN - 1, npad - 1);
-    const int i0 = min(max(o0 - 1, 0), npad - TR);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
-    const int tw = wave / KS, ks = wave % KS;
-    const int j0 = (ng * TPW + tw) * 16;
+    const int o0 = FUSED_OWN * mb;
+    const int o1 = min(o0 + FUSED_OWN - 1, npad - 1);
+    const int i0 = min(max(o0 - 1, 0), npad - 16);
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ks = wave;
+    const int j0 = ng * 16;
+    const int t = (threadIdx.x >> 4) & 15, cc = threadIdx.x & 15;
+    const int k = i0 + t;
+    const bool own = k >= o0 && k <= o1;
 
-    // epilogue operands (base rows of this lane's tile, and for this thread's output elements the
-    // next source and the three coupling coefficients) are loaded before the GEMM so that their
-    // latency overlaps it instead of trailing it.  Element q of this thread is tile row
-    // (threadIdx.x / 16) + 16 (q % RB), column threadIdx.x % 16 of RHS tile q / RB.
-    cf pbase[RB][4];
+    // operand staging: per wave A[k][i] and B[k][j] of one k-block
+    __shared__ cf ops_lds[4][2][16][STR];
+    cf(*LA)[STR] = ops_lds[wave][0];
+    cf(*LB)[STR] = ops_lds[wave][1];
+    const int64_t a_off = (ALAY == A_COL) ? i0 : (int64_t)i0 * npad;
+    const auto ra = __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(tk.X + a_off), (short)0,
+                                                      (int)(((int64_t)npad * npad - a_off) * 8), 0x00020000);
+    const auto rbuf = __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(tk.B + j0), (short)0,
+                                                        (int)(((int64_t)npad * Rpad - j0) * 8), 0x00020000);
+    // lane l fetches 16 bytes at line (l / 8) + 8 q, elements 2 (l % 8) + {0, 1} of the line; a line
+    // is a k of A_COL and B (16 rows / columns), a row of A_CONJT (16 k)
+    const int ln = lane >> 3, el = 2 * (lane & 7);
+    const uint32_t va = (uint32_t)((ln * npad + el) * 8), vb = (uint32_t)((ln * Rpad + el) * 8);
+    struct Stage {
+        v4f a[2], b[2];
+    };
+    auto gload = [&](Stage &r, int k0) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t sa = (ALAY == A_COL) ? (uint32_t)((k0 + 8 * q) * npad * 8) : (uint32_t)((8 * q * npad + k0) * 8);
+            r.a[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(ra, va, sa, 0));
+            r.b[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rbuf, vb, (uint32_t)((k0 + 8 * q) * Rpad * 8), 0));
+        }
+    };
+    auto stage = [&](const Stage &r) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int ln8 = ln + 8 * q;
+            if (ALAY == A_COL) {
+                LA[ln8][el] = mk(r.a[q].x, r.a[q].y);
+                LA[ln8][el + 1] = mk(r.a[q].z, r.a[q].w);
+            } else {   // row ln8 of A, k = el, el + 1: conjugated into LA[k][row]
+                LA[el][ln8] = mk(r.a[q].x, -r.a[q].y);
+                LA[el + 1][ln8] = mk(r.a[q].z, -r.a[q].w);
+            }
+            LB[ln8][el] = mk(r.b[q].x, r.b[q].y);
+            LB[ln8][el + 1] = mk(r.b[q].z, r.b[q].w);
+        }
+    };
+    using Acc = CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH, SWEEP_FOLD>;
+    Acc acc;
+    acc.zero();
+    {
+        const int nkb = npad / 16;
+        const int nk = (nkb - ks + KS - 1) / KS;
+        const int last = ks + (nk - 1) * KS;
+        auto k0 = [&](int u) {
+            const int kb = ks + u * KS;
+            return 16 * (kb < last ? kb : last);
+        };
+        Stage st[DEPTH + 1];
+#pragma unroll
+        for (int u = 0; u < DEPTH; ++u) gload(st[u], k0(u));
+        for (int t0 = 0; t0 < nk; t0 += DEPTH + 1) {
+#pragma unroll
+            for (int u = 0; u <= DEPTH; ++u) {
+                gload(st[(u + DEPTH) % (DEPTH + 1)], k0(t0 + u + DEPTH));
+                __builtin_amdgcn_sched_barrier(0);
+                if (t0 + u < nk) {
+                    stage(st[u]);
+                    typename Acc::Ops o;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        o.a[0][s] = LA[4 * g + s][c];
+                        o.b[0][s] = LB[4 * g + s][c];
+                    }
+                    acc.mma(o);
+                    acc.fold();
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+
+    // epilogue operands, loaded after the GEMM (keeps the loop's registers free)
+    cf pbase[4];
+    cf pn = mk(0.f, 0.f), pc = mk(0.f, 0.f), ps = mk(0.f, 0.f), psrc = mk(0.f, 0.f);
     if (tk.base && ks == 0) {
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) pbase[rb][r] = tk.base[(int64_t)(i0 + 16 * rb + 4 * g + r) * Rpad + j0 + c];
+        for (int r = 0; r < 4; ++r) pbase[r] = tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
     }
-    const int cc = threadIdx.x & 15;
-    bool own[RB];
-    cf pn[RB], pc[RB], ps[RB], psrc[EPT];
-#pragma unroll
-    for (int rr = 0; rr < RB; ++rr) {
-        const int k = i0 + (threadIdx.x >> 4) + 16 * rr;
-        own[rr] = k >= o0 && k <= o1;
-        pn[rr] = pc[rr] = ps[rr] = mk(0.f, 0.f);
-        if (own[rr] && tk.Bn) {
-            if (!tk.transposed) {
-                pc[rr] = tk.cc[k];
-                if (k >= 1) pn[rr] = tk.cn[k];
-                if (k + 1 < npad) ps[rr] = tk.cs[k];
-            } else {
-                pc[rr] = conjf_(tk.cc[k]);
-                if (k >= 1) pn[rr] = conjf_(tk.cs[k - 1]);
-                if (k + 1 < npad) ps[rr] = conjf_(tk.cn[k + 1]);
-            }
-#pragma unroll
-            for (int tq = 0; tq < TPW; ++tq)
-                psrc[rr + RB * tq] = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + (ng * TPW + tq) * 16 + cc] : mk(0.f, 0.f);
-        }
-    }
-
-    CgemmWave<RB, 1, ALAY, B_ROW, false, DEPTH, SWEEP_FOLD> acc;
-    acc.zero();
-    const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
-    acc.run(A, npad, tk.B + j0, Rpad, ks, npad / 16, KS, g, c);
-
-    __shared__ float4 red[4][RB][2][64];
-    __shared__ cf tile[TPW][TR][17];
-    float re[RB][4], im[RB][4];
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb) acc.result(rb, 0, re[rb], im[rb]);
-    if (KS > 1) {
-        if (ks > 0) {
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                red[wave][rb][0][lane] = make_float4(re[rb][0], re[rb][1], re[rb][2], re[rb][3]);
-                red[wave][rb][1][lane] = make_float4(im[rb][0], im[rb][1], im[rb][2], im[rb][3]);
-            }
-        }
-        __syncthreads();
-    }
-    if (ks == 0) {
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-#pragma unroll
-            for (int w = 1; w < KS; ++w) {
-                const float4 u = red[wave + w][rb][0][lane], v = red[wave + w][rb][1][lane];
-                re[rb][0] += u.x; re[rb][1] += u.y; re[rb][2] += u.z; re[rb][3] += u.w;
-                im[rb][0] += v.x; im[rb][1] += v.y; im[rb][2] += v.z; im[rb][3] += v.w;
-            }
-            // lane holds tile rows 16 rb + 4 g + r of column j0 + c
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                cf v = mk(re[rb][r], im[rb][r]);
-                if (tk.base) v = v + pbase[rb][r];
-                tile[tw][16 * rb + 4 * g + r][c] = v;
-            }
-        }
-    }
-    __syncthreads();
-    // epilogue: this thread's elements
-#pragma unroll
-    for (int q = 0; q < EPT; ++q) {
-        const int rr = q % RB, tq = q / RB;
-        if (!own[rr]) continue;
-        const int t = (threadIdx.x >> 4) + 16 * rr, k = i0 + t;
-        const int64_t off = (int64_t)k * Rpad + (ng * TPW + tq) * 16 + cc;
-        const cf v = tile[tq][t][cc];
-        tk.out[off] = v;
-        if (!tk.Bn) continue;
-        const cf vn = (k >= 1) ? tile[tq][t - 1][cc] : mk(0.f, 0.f);        // row k - 1 (in the tile when k >= 1)
-        const cf vs = (k + 1 < npad) ? tile[tq][t + 1][cc] : mk(0.f, 0.f);  // row k + 1 (in the tile when < npad)
-        // the same products and order as operand_b: P v = c_c v_k + c_n v_{k-1} + c_s v_{k+1}
-        // (transposed: conj(cc_k) v_k + conj(cn_{k+1}) v_{k+1} + conj(cs_{k-1}) v_{k-1})
-        cf a = pc[rr] * v;
-        if (!tk.transposed) {
-            if (k >= 1) cfma(a, pn[rr], vn);
-            if (k + 1 < npad) cfma(a, ps[rr], vs);
-        } else {
-            if (k + 1 < npad) cfma(a, ps[rr], vs);
-            if (k >= 1) cfma(a, pn[rr], vn);
-        }
-        tk.Bn[off] = psrc[q] - a;
-    }
-}
-
-// ---- split-K step on 32 x 32 tiles with operands shared through LDS ------------------------------
-//
-// The 16-row steps above are bound by operand traffic from L2: each wave streams its own A (16 rows of
-// the page) and B (16 RHS of the operand) over its K share, so every page row is read once per RHS
-// tile and every operand row once per strip (~170 MB per step at 576 x 256 against ~8 MB of distinct
-// data).  Here a workgroup's four waves compute the four 16 x 16 blocks of one 32-row (30 owned) x
-// 32-RHS tile over one quarter of K, with each k-block of A (32 rows) and B (32 RHS) staged once in
-// LDS and read by the two waves that share it: half the traffic per output element.  Splitting K
-// over SK_PARTS workgroups keeps the launch at ~4.75 waves per SIMD (1216 workgroups at 558 rows and
-// 256 RHS).  Each workgroup writes its FP32-rounded partial tile (the k-block fold's FP64 sums, as
-// the other variants round them before their LDS reduction); the last of a tile's workgroups to
-// arrive (an atomic count per tile) sums the SK_PARTS partials in part order in FP64, so the result
-// does not depend on the arrival order, and runs the fused epilogue.  The workgroups of one tile
-// are dealt to the same XCD (blockIdx % 8), whose L2 is their point of coherence: each waits for
-// its partial's stores to complete before counting itself in, and the last one invalidates its L1
-// before reading the others' partials.  No workgroup ever waits for another.
-constexpr int SK_PARTS = 4;
-constexpr int SK_TR = 32, SK_OWN = 30, SK_TC = 32;
-
-__device__ __forceinline__ void wait_vmem_stores() {
-#if defined(__HIP_DEVICE_COMPILE__)
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-#endif
-}
-__device__ __forceinline__ void invalidate_l1() {
-#if defined(__HIP_DEVICE_COMPILE__)
-    asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
-#endif
-}
-
-template <int ALAY>
-__device__ __forceinline__ void sk_load(const cf *X, const cf *B, int npad, int Rpad, int i0, int j0, int kb, int t,
-                                        cf (&a)[2], cf (&b)[2]) {
-    const int k0 = 16 * kb;
-    if (ALAY == A_COL) {   // A[i][k] = X[k npad + i]: thread t takes rows 2 (t % 16) + {0, 1} of k = t / 16
-        const cf *src = X + (int64_t)(k0 + (t >> 4)) * npad + i0 + 2 * (t & 15);
-        a[0] = src[0];
-        a[1] = src[1];
-    } else {               // A[i][k] = conj(X[i npad + k]): thread t takes k = 2 (t % 8) + {0, 1} of row t / 8
-        const float4 u = *reinterpret_cast<const float4 *>(X + (int64_t)(i0 + (t >> 3)) * npad + k0 + 2 * (t & 7));
-        a[0] = mk(u.x, -u.y);
-        a[1] = mk(u.z, -u.w);
-    }
-    const float4 v = *reinterpret_cast<const float4 *>(B + (int64_t)(k0 + (t >> 4)) * Rpad + j0 + 2 * (t & 15));
-    b[0] = mk(v.x, v.y);
-    b[1] = mk(v.z, v.w);
-}
-
-// LDS staging: As[k][row], Bs[k][col] of one k-block (k-major, so a lane's MFMA operand reads are
-// contiguous over its 16 rows / columns)
-template <int ALAY>
-__device__ __forceinline__ void sk_stage(cf (*As)[SK_TR], cf (*Bs)[SK_TC], int t, const cf (&a)[2], const cf (&b)[2]) {
-    if (ALAY == A_COL) {
-        *reinterpret_cast<float4 *>(&As[t >> 4][2 * (t & 15)]) = make_float4(a[0].re, a[0].im, a[1].re, a[1].im);
-    } else {
-        As[2 * (t & 7)][t >> 3] = a[0];
-        As[2 * (t & 7) + 1][t >> 3] = a[1];
-    }
-    *reinterpret_cast<float4 *>(&Bs[t >> 4][2 * (t & 15)]) = make_float4(b[0].re, b[0].im, b[1].re, b[1].im);
-}
-
-template <int ALAY>
-__global__ void __launch_bounds__(256) sweep_splitk_kernel(FusedParams p) {
-    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
-    const int tile = xcd * p.per_xcd + (local / SK_PARTS);
-    const int part = local % SK_PARTS;
-    if (tile >= p.units) return;
-    const int npad = p.npad, Rpad = p.Rpad;
-    const int Ng = Rpad / SK_TC;
-    const int ng = tile % Ng;
-    const int mb = (tile / Ng) % p.strips;
-    const int task = tile / (Ng * p.strips);
-    const FusedTask &tk = p.task[task];
-    const int o0 = SK_OWN * mb;
-    const int o1 = min(o0 + SK_OWN - 1, npad - 1);
-    const int i0 = min(max(o0 - 1, 0), npad - SK_TR);
-    const int j0 = ng * SK_TC;
-    const int nkb = npad / 16, per = (nkb + SK_PARTS - 1) / SK_PARTS;
-    const int kb0 = min(part * per, nkb), kb1 = min(kb0 + per, nkb);
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, g = lane >> 4, c = lane & 15;
-    const int rb = wave >> 1, cb = wave & 1;   // this wave's 16 x 16 block of the 32 x 32 tile
-
-    // one LDS block: the two k-block buffers of A and of B, later the 32 x 33 result tile
-    __shared__ cf sk_lds[2 * 16 * SK_TR + 2 * 16 * SK_TC];
-    __shared__ int last_flag;
-    cf(*As)[16][SK_TR] = reinterpret_cast<cf(*)[16][SK_TR]>(sk_lds);
-    cf(*Bs)[16][SK_TC] = reinterpret_cast<cf(*)[16][SK_TC]>(sk_lds + 2 * 16 * SK_TR);
-    v4f p1 = {0.f, 0.f, 0.f, 0.f}, p3 = p1;
-    double q1[4] = {0.0, 0.0, 0.0, 0.0}, q3[4] = {0.0, 0.0, 0.0, 0.0};
-    cf ra[2], rbv[2];
-    if (kb0 < kb1) {
-        sk_load<ALAY>(tk.X, tk.B, npad, Rpad, i0, j0, kb0, t, ra, rbv);
-        sk_stage<ALAY>(As[0], Bs[0], t, ra, rbv);
-    }
-    __syncthreads();
-    for (int kb = kb0; kb < kb1; ++kb) {
-        const int cur = (kb - kb0) & 1;
-        const bool more = kb + 1 < kb1;
-        if (more) sk_load<ALAY>(tk.X, tk.B, npad, Rpad, i0, j0, kb + 1, t, ra, rbv);
-#pragma unroll
-        for (int s4 = 0; s4 < 4; ++s4) {
-            const cf x = As[cur][4 * g + s4][16 * rb + c], y = Bs[cur][4 * g + s4][16 * cb + c];
-            p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.re, p1, 0, 0, 0);
-            p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.re, y.im, p3, 0, 0, 0);
-            p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-x.im, y.im, p1, 0, 0, 0);
-            p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(x.im, y.re, p3, 0, 0, 0);
-        }
-        // fold the k-block into the FP64 sums
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            q1[r] += (double)p1[r];
-            q3[r] += (double)p3[r];
-        }
-        p1 = v4f{0.f, 0.f, 0.f, 0.f};
-        p3 = p1;
-        if (more) sk_stage<ALAY>(As[cur ^ 1], Bs[cur ^ 1], t, ra, rbv);
-        __syncthreads();
-    }
-    // this workgroup's partial: lane holds rows 16 rb + 4 g + r, column 16 cb + c
-    cf *mine = p.skpart + ((int64_t)tile * SK_PARTS + part) * (SK_TR * SK_TC);
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-        mine[(16 * rb + 4 * g + r) * SK_TC + 16 * cb + c] = mk((float)q1[r], (float)q3[r]);
-    wait_vmem_stores();
-    __syncthreads();
-    if (t == 0) {
-        const int old = __hip_atomic_fetch_add(p.skcount + tile, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last_flag = (old % SK_PARTS) == SK_PARTS - 1;
-    }
-    __syncthreads();
-    if (!last_flag) return;
-    invalidate_l1();
-    // the tile: the partials summed in part order in FP64 (+ base), staged for the epilogue
-    cf(*tile_s)[SK_TC + 1] = reinterpret_cast<cf(*)[SK_TC + 1]>(sk_lds);   // 32 x 33 cf within the LDS block
-    const cf *parts = p.skpart + (int64_t)tile * SK_PARTS * (SK_TR * SK_TC);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int row = 16 * rb + 4 * g + r, col = 16 * cb + c;
-        double sr = 0.0, si = 0.0;
-#pragma unroll
-        for (int q = 0; q < SK_PARTS; ++q) {
-            const cf v = parts[q * (SK_TR * SK_TC) + row * SK_TC + col];
-            sr += (double)v.re;
-            si += (double)v.im;
-        }
-        cf v = mk((float)sr, (float)si);
-        if (tk.base) v = v + tk.base[(int64_t)(i0 + row) * Rpad + j0 + col];
-        tile_s[row][col] = v;
-    }
-    __syncthreads();
-    // epilogue: thread t handles column t % 32 of rows t / 32 + 8 q
-    const int cc = t & 31;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int row = (t >> 5) + 8 * q, k = i0 + row;
-        if (k < o0 || k > o1) continue;
-        const int64_t off = (int64_t)k * Rpad + j0 + cc;
-        const cf v = tile_s[row][cc];
-        tk.out[off] = v;
-        if (!tk.Bn) continue;
-        cf pn = mk(0.f, 0.f), pc, ps = mk(0.f, 0.f);
+    if (own && tk.Bn) {
         if (!tk.transposed) {
             pc = tk.cc[k];
             if (k >= 1) pn = tk.cn[k];
@@ -587,42 +427,59 @@ __global__ void __launch_bounds__(256) sweep_splitk_kernel(FusedParams p) {
             if (k >= 1) pn = conjf_(tk.cs[k - 1]);
             if (k + 1 < npad) ps = conjf_(tk.cn[k + 1]);
         }
-        const cf vn = (k >= 1) ? tile_s[row - 1][cc] : mk(0.f, 0.f);
-        const cf vs = (k + 1 < npad) ? tile_s[row + 1][cc] : mk(0.f, 0.f);
-        // the same products and order as operand_b
-        cf a = pc * v;
-        if (!tk.transposed) {
-            if (k >= 1) cfma(a, pn, vn);
-            if (k + 1 < npad) cfma(a, ps, vs);
-        } else {
-            if (k + 1 < npad) cfma(a, ps, vs);
-            if (k >= 1) cfma(a, pn, vn);
-        }
-        const cf src = tk.nsrc ? tk.nsrc[off] : mk(0.f, 0.f);
-        tk.Bn[off] = src - a;
+        psrc = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + j0 + cc] : mk(0.f, 0.f);
     }
+
+    __shared__ float4 red[4][2][64];
+    __shared__ cf tile[16][17];
+    float re[4], im[4];
+    acc.result(0, 0, re, im);
+    if (ks > 0) {
+        red[wave][0][lane] = make_float4(re[0], re[1], re[2], re[3]);
+        red[wave][1][lane] = make_float4(im[0], im[1], im[2], im[3]);
+    }
+    __syncthreads();
+    if (ks == 0) {
+#pragma unroll
+        for (int w = 1; w < KS; ++w) {
+            const float4 u = red[wave + w][0][lane], v = red[wave + w][1][lane];
+            re[0] += u.x; re[1] += u.y; re[2] += u.z; re[3] += u.w;
+            im[0] += v.x; im[1] += v.y; im[2] += v.z; im[3] += v.w;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cf v = mk(re[r], im[r]);
+            if (tk.base) v = v + pbase[r];
+            tile[4 * g + r][c] = v;
+        }
+    }
+    __syncthreads();
+    if (!own) return;
+    const int64_t off = (int64_t)k * Rpad + j0 + cc;
+    const cf v = tile[t][cc];
+    tk.out[off] = v;
+    if (!tk.Bn) return;
+    const cf vn = (k >= 1) ? tile[t - 1][cc] : mk(0.f, 0.f);
+    const cf vs = (k + 1 < npad) ? tile[t + 1][cc] : mk(0.f, 0.f);
+    // the same products and order as operand_b (see sweep_fused_body)
+    cf a = pc * v;
+    if (!tk.transposed) {
+        if (k >= 1) cfma(a, pn, vn);
+        if (k + 1 < npad) cfma(a, ps, vs);
+    } else {
+        if (k + 1 < npad) cfma(a, ps, vs);
+        if (k >= 1) cfma(a, pn, vn);
+    }
+    tk.Bn[off] = psrc - a;
 }
 
-// Clears the split-K arrival counts (a fill kernel, so the clear is an ordinary launch).
-__global__ void sk_clear_kernel(int *count, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) count[i] = 0;
-}
-
-// 1280 workgroups of four waves (40 strips x 16 RHS tiles x 2 chains at 558 rows and 256 RHS)
-// need five resident waves per SIMD to run in one round; with two k-blocks in flight the step fits
-// that register budget without spilling (deeper prefetch does not).  Launches that take several
-// rounds anyway (larger blocks or more RHS) run faster with the unconstrained build, so both are
-// autotuner variants.
-template <int ALAY, int KS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
-sweep_fused_kernel_d2(FusedParams p) {
-    sweep_fused_body<ALAY, 2, KS>(p);
-}
-template <int ALAY, int DEPTH, int KS, int TR = 16>
+template <int ALAY, int DEPTH, bool LATE, bool BUF>
 __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
-    if constexpr (TR == 16) sweep_fused_body<ALAY, DEPTH, KS>(p);
-    else sweep_fused_body_tall<ALAY, DEPTH, KS, TR>(p);
+    sweep_fused_body<ALAY, DEPTH, LATE, BUF>(p);
+}
+template <int ALAY, int DEPTH>
+__global__ void __launch_bounds__(256) sweep_fused_kernel_lds(FusedParams p) {
+    sweep_fused_body_lds<ALAY, DEPTH>(p);
 }
 
 // Zeroes rows r0 .. npad-1 of the [npad][Rpad] blocks of two arrays (blockIdx.y selects the
@@ -639,39 +496,32 @@ __global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r
     dst[blkno * npad * Rpad + (int64_t)r0 * Rpad + e] = mk(0.f, 0.f);
 }
 
-// Kernel variants of the fused step and their per-shape choice.  Variant v: (DEPTH, KS, tile
-// rows) = (2, 4, 16) five waves, (3, 4, 16), (4, 2, 16), (2, 2, 16) five waves, (2, 4, 16),
-// (2, 2, 16), (2, 4, 32), (3, 4, 32), (2, 4, 64); a variant with TPW tiles per workgroup needs
-// Rpad / 16 to be a multiple of TPW.
-constexpr int N_FUSED_VARIANTS = 10;
-constexpr int SPLITK_VARIANT = 9;   // 32 x 32 tiles, K over SK_PARTS workgroups, operands through LDS
-constexpr int FUSED_KS[N_FUSED_VARIANTS] = {4, 4, 2, 2, 4, 2, 4, 4, 4, 4};
-constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {2, 3, 4, 2, 2, 2, 2, 3, 2, 1};
-constexpr bool FUSED_FIVE[N_FUSED_VARIANTS] = {true, false, false, true, false, false, false, false, false, false};
-constexpr int FUSED_TR[N_FUSED_VARIANTS] = {16, 16, 16, 16, 16, 16, 32, 32, 64, 32};
+// Kernel variants of the fused step and their per-shape choice.  All take 16-row tiles with K dealt
+// over the four waves of a workgroup in the same order, so they give bit-identical results; they
+// differ in how the operands reach the MFMA:
+//   0, 1  16-byte loads of whole lines staged through a wave-private LDS tile, DEPTH 1 / 2
+//   2     buffer loads into registers, epilogue operands loaded after the GEMM, DEPTH 1
+//   3     buffer loads into registers, DEPTH 2
+//   4     flat loads into registers, DEPTH 2
+constexpr int N_FUSED_VARIANTS = 5;
+constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {1, 2, 1, 2, 2};
+const char *const FUSED_KIND[N_FUSED_VARIANTS] = {"LDS-staged", "LDS-staged", "buffer loads, late epilogue operands",
+                                                  "buffer loads", "flat loads"};
 
-// strips of variant v's tiles over n rows (each owns FUSED_TR - 2 rows), and the rows they cover
-inline int fused_strips(int v, int n) { return (n + FUSED_TR[v] - 3) / (FUSED_TR[v] - 2); }
-inline int fused_covered(int v, int n, int npad) { return std::min(fused_strips(v, n) * (FUSED_TR[v] - 2), npad); }
+// strips of 16-row tiles over n rows (each owns FUSED_OWN rows), and the rows they cover
+inline int fused_strips(int n) { return (n + FUSED_OWN - 1) / FUSED_OWN; }
+inline int fused_covered(int n, int npad) { return std::min(fused_strips(n) * FUSED_OWN, npad); }
 
-// Workgroup geometry of variant v for fp's shape: strips, units (tiles; for the split-K variant one tile
-// per SK_PARTS workgroups) and units per XCD slot.
-void fused_geometry(int v, FusedParams &fp) {
-    fp.strips = fused_strips(v, fp.n);
-    if (v == SPLITK_VARIANT) {
-        fp.units = fp.ntasks * fp.strips * (fp.Rpad / SK_TC);
-        fp.per_xcd = (fp.units + 7) / 8;
-        return;
-    }
-    const int tpw = 4 / FUSED_KS[v];
-    fp.units = fp.ntasks * fp.strips * (fp.Rpad / 16) / tpw;
+// Workgroup geometry for fp's shape: strips, units (tiles) and units per XCD slot.
+void fused_geometry(FusedParams &fp) {
+    fp.strips = fused_strips(fp.n);
+    fp.units = fp.ntasks * fp.strips * (fp.Rpad / 16);
     fp.per_xcd = (fp.units + 7) / 8;
 }
-inline int splitk_tiles(int n, int Rpad) { return 2 * ((n + SK_OWN - 1) / SK_OWN) * (Rpad / SK_TC); }
 
 template <int ALAY>
 void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool profiled) {
-    const dim3 grid(8 * fp.per_xcd * (v == SPLITK_VARIANT ? SK_PARTS : 1)), block(256);
+    const dim3 grid(8 * fp.per_xcd), block(256);
     auto go = [&](auto kernel) {
         if (profiled) launch_k(PC_SWEEP_GEMM, kernel, grid, block, 0, s, fp);
         else {
@@ -680,16 +530,11 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
         }
     };
     switch (v) {
-        case 1: go(sweep_fused_kernel<ALAY, 3, 4>); break;
-        case 2: go(sweep_fused_kernel<ALAY, 4, 2>); break;
-        case 3: go(sweep_fused_kernel_d2<ALAY, 2>); break;
-        case 4: go(sweep_fused_kernel<ALAY, 2, 4>); break;
-        case 5: go(sweep_fused_kernel<ALAY, 2, 2>); break;
-        case 6: go(sweep_fused_kernel<ALAY, 2, 4, 32>); break;
-        case 7: go(sweep_fused_kernel<ALAY, 3, 4, 32>); break;
-        case 8: go(sweep_fused_kernel<ALAY, 2, 4, 64>); break;
-        case SPLITK_VARIANT: go(sweep_splitk_kernel<ALAY>); break;
-        default: go(sweep_fused_kernel_d2<ALAY, 4>); break;
+        case 0: go(sweep_fused_kernel_lds<ALAY, 1>); break;
+        case 1: go(sweep_fused_kernel_lds<ALAY, 2>); break;
+        case 2: go(sweep_fused_kernel<ALAY, 1, true, true>); break;
+        case 3: go(sweep_fused_kernel<ALAY, 2, false, true>); break;
+        default: go(sweep_fused_kernel<ALAY, 2, false, false>); break;
     }
 }
 
@@ -702,15 +547,15 @@ struct FusedKey {
 std::mutex g_fused_mu;
 std::map<FusedKey, int> g_fused_choice;
 
-bool fused_eligible(int v, const FusedParams &fp) {
-    if (v == SPLITK_VARIANT) return fp.skpart && fp.skcount && fp.Rpad % SK_TC == 0 && SK_TR <= fp.npad;
-    return (fp.Rpad / 16) % (4 / FUSED_KS[v]) == 0 && FUSED_TR[v] <= fp.npad;
-}
+bool fused_eligible(int v, const FusedParams &fp) { return v >= 0 && v < N_FUSED_VARIANTS && fp.npad >= 16; }
 
 // Times every eligible variant on `fp` (a real step whose outputs the sweep rewrites before
-// reading them) and returns the fastest; the choice is kept per (npad, Rpad, adjoint).
+// reading them) and returns the fastest; the choice is kept per (npad, Rpad, adjoint).  Within a
+// burst, launch b reads the pages b steps further along each task's chain (task 0 walks up from its
+// page, task 1 down, as the left and right eliminations do), so each launch meets its pages in HBM
+// as the sweep does, not in the L2 that the previous launch filled.
 // WUST_SWEEP_VARIANT=v forces variant v where it is eligible (read per solve, for tests).
-int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
+int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s, const Operator &op) {
     if (const char *e = std::getenv("WUST_SWEEP_VARIANT"); e && e[0]) {
         const int v = std::atoi(e);
         if (v >= 0 && v < N_FUSED_VARIANTS && fused_eligible(v, fp)) return v;
@@ -729,28 +574,39 @@ int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
     // Each eligible variant runs BURST back-to-back launches (as the sweep runs them) per round,
     // rounds interleaving the variants; a variant's time is its best round (the least disturbed).
     constexpr int ROUNDS = 3, BURST = 8;
+    const int64_t page_elems = (int64_t)op.npad * op.npad;
+    auto walked = [&](const FusedParams &base, int b) {
+        FusedParams w = base;
+        for (int i = 0; i < w.ntasks; ++i) {
+            const int64_t j0 = (w.task[i].X - op.pages) / page_elems;
+            const int64_t j = std::min<int64_t>(std::max<int64_t>(j0 + (i == 0 ? b : -b), 0), op.m - 1);
+            w.task[i].X = op.page((int)j);
+        }
+        return w;
+    };
     hipEvent_t e0, e1;
     WUST_HIP_CHECK(hipEventCreate(&e0));
     WUST_HIP_CHECK(hipEventCreate(&e1));
     FusedParams q[N_FUSED_VARIANTS];
     bool ok[N_FUSED_VARIANTS];
     float per_launch[N_FUSED_VARIANTS];
-    auto run = [&](int v) {
-        if (adjoint) launch_fused_variant<A_CONJT>(v, q[v], s, false);
-        else launch_fused_variant<A_COL>(v, q[v], s, false);
+    auto run = [&](int v, int b) {
+        const FusedParams w = walked(q[v], b);
+        if (adjoint) launch_fused_variant<A_CONJT>(v, w, s, false);
+        else launch_fused_variant<A_COL>(v, w, s, false);
     };
     for (int v = 0; v < N_FUSED_VARIANTS; ++v) {
         ok[v] = fused_eligible(v, fp);
         per_launch[v] = 1e30f;
         q[v] = fp;
-        fused_geometry(v, q[v]);
-        if (ok[v]) run(v);    // warms the code object and the caches
+        fused_geometry(q[v]);
+        if (ok[v]) run(v, 0);    // warms the code object
     }
     for (int round = 0; round < ROUNDS; ++round)
         for (int v = 0; v < N_FUSED_VARIANTS; ++v) {
             if (!ok[v]) continue;
             WUST_HIP_CHECK(hipEventRecord(e0, s));
-            for (int b = 0; b < BURST; ++b) run(v);
+            for (int b = 0; b < BURST; ++b) run(v, round * BURST + b);
             WUST_HIP_CHECK(hipEventRecord(e1, s));
             WUST_HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -768,9 +624,9 @@ int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s) {
         for (int v = 0; v < N_FUSED_VARIANTS; ++v)
             if (ok[v]) std::fprintf(stderr, " [%d] %.2f", v, 1e3f * per_launch[v]);
         std::fprintf(stderr, "\n");
-        std::fprintf(stderr, "sweep step kernel: npad %d Rpad %d %s -> %d-row tiles, depth %d, K split %d%s (%.2f us)\n",
-                     fp.npad, fp.Rpad, adjoint ? "adjoint" : "forward", FUSED_TR[best], FUSED_DEPTH[best],
-                     FUSED_KS[best], FUSED_FIVE[best] ? ", five waves" : "", 1e3f * per_launch[best]);
+        std::fprintf(stderr, "sweep step kernel: npad %d Rpad %d %s -> variant %d (%s, depth %d, %.2f us)\n", fp.npad,
+                     fp.Rpad, adjoint ? "adjoint" : "forward", best, FUSED_KIND[best], FUSED_DEPTH[best],
+                     1e3f * per_launch[best]);
     }
     std::lock_guard<std::mutex> lk(g_fused_mu);
     g_fused_choice[key] = best;
@@ -792,12 +648,6 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
     const int64_t blk = (int64_t)npad * Rpad;
     cf *yf = work;                                 // forward iterates y_j / z_j: [m][npad][Rpad]
     cf *ops = work + (int64_t)m * blk;             // operand blocks: [2 chains][2 ping-pong]
-    // split-K variant: partial tiles, then the arrival counts (cleared per solve)
-    cf *sk_part = work + ((int64_t)m + 4) * blk;
-    const int sk_tiles = splitk_tiles(op.n, Rpad);
-    int *sk_count = reinterpret_cast<int *>(sk_part + (int64_t)sk_tiles * SK_PARTS * SK_TR * SK_TC);
-    hipLaunchKernelGGL(sk_clear_kernel, dim3((sk_tiles + 255) / 256), dim3(256), 0, s, sk_count, sk_tiles);
-    WUST_LAUNCH_CHECK();
     if (unfused) {
         solve_engine_unfused(op, src, out, ops, Rpad, adjoint, s);
         return;
@@ -811,8 +661,7 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
     // strips cover the n real rows only (1280 rather than 1344 workgroups at 558 rows, 256 RHS and
     // 16-row tiles: five per CU); the padding rows beyond what every variant's strips cover are
     // zeroed once per solve
-    int covered = npad;
-    for (int v = 0; v < N_FUSED_VARIANTS; ++v) covered = std::min(covered, fused_covered(v, op.n, npad));
+    const int covered = fused_covered(op.n, npad);
     if (covered < npad) {
         const int64_t per = (int64_t)(npad - covered) * Rpad, most = per * (m + 4);
         hipLaunchKernelGGL(pad_rows_zero_kernel, dim3((unsigned)((most + 255) / 256), 2), dim3(256), 0, s, work,
@@ -834,11 +683,8 @@ void solve_engine(const Operator &op, const cf *src, cf *out, cf *work, int Rpad
         fp.n = op.n;
         fp.npad = npad;
         fp.Rpad = Rpad;
-        fp.skpart = sk_part;
-        fp.skcount = sk_count;
-        fp.strips = fused_strips(0, op.n);
-        if (variant < 0) variant = fused_variant(fp, adjoint, s);
-        fused_geometry(variant, fp);
+        fused_geometry(fp);
+        if (variant < 0) variant = fused_variant(fp, adjoint, s, op);
         prof_units(PC_SWEEP_GEMM, fp.ntasks);
         if (adjoint) launch_fused_variant<A_CONJT>(variant, fp, s, true);
         else launch_fused_variant<A_COL>(variant, fp, s, true);
