@@ -489,7 +489,7 @@ __device__ __forceinline__ void ns_tile(int r, int T, int &ti, int &tj) {
 // round-robin) and each accumulates the whole tile as 2x2 blocks of 16x16
 // (CgemmWave); the partial tiles are summed through LDS.
 template <bool M3>
-__global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) ns_gemm_kernel(NsParams p) {
     const int T = p.npad / NB, per_task = T * T, G = p.ntasks * per_task;
     const int per_xcd = (G + 7) / 8;
     const int L = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
@@ -526,7 +526,9 @@ __global__ void __launch_bounds__(256) ns_gemm_kernel(NsParams p) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform (scalar k offsets)
     // the residual E = I - T X (four real products) folds each k-block into FP64 sums: its accuracy
     // sets the accuracy of the inverse that the update X (I + E) produces
-    CgemmWave<2, 2, A_COL, B_KCOL, M3, 2, !M3> acc;
+    // (three workgroups per CU hold a 576^2 product's 648 tiles in one round: the FP64 sums of the
+    // four-product form leave room for one k-block of operands in flight, the 3M form for two)
+    CgemmWave<2, 2, A_COL, B_KCOL, M3, (M3 ? 2 : 1), !M3> acc;
     acc.zero();
     const int64_t nn = (int64_t)ld * ld;
     acc.run_buf(A + i0, ld, nn - i0, B + (int64_t)j0 * ld, ld, nn - (int64_t)j0 * ld, wave, ld / 16, 4, g, c);
