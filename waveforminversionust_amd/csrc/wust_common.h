@@ -119,7 +119,20 @@ struct Error : public std::runtime_error {
             throw ::wust::Error(WUST_HIP_ERROR, std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
 
+#ifdef WUST_DEBUG_SYNC
+// Debug build: every launch is followed by a device synchronization, so an asynchronous failure
+// (a fault, an abort) is reported at the launch that caused it, with its file and line.
+#define WUST_LAUNCH_CHECK()                                                                            \
+    do {                                                                                               \
+        WUST_HIP_CHECK(hipGetLastError());                                                             \
+        const hipError_t _s = hipDeviceSynchronize();                                                  \
+        if (_s != hipSuccess)                                                                          \
+            throw ::wust::Error(WUST_HIP_ERROR, std::string(__FILE__ ":") + std::to_string(__LINE__) + \
+                                                    ": kernel failed: " + hipGetErrorString(_s));      \
+    } while (0)
+#else
 #define WUST_LAUNCH_CHECK() WUST_HIP_CHECK(hipGetLastError())
+#endif
 
 inline void require(bool ok, const std::string &msg) {
     if (!ok) throw Error(WUST_INVALID_INPUT, msg);
