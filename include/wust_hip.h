@@ -64,6 +64,14 @@ typedef enum {
  * device-side residual bound; if any block misses the bound the whole
  * factor is redone by Gauss-Jordan (wust_info.factor_fallback). */
 #define WUST_FACTOR_GAUSS_JORDAN 0x100
+/* Flag OR-ed into the mode of wust_helmholtz_create (engine modes only, no
+ * reference counterpart): Newton-Schulz with the default round budgets.  Without
+ * it, the round budget of each chain step is learned from the previous
+ * factorization of an operator with the same shape, frequency and grid (a
+ * process-wide profile), so the c64 factor -- and the rounding of everything
+ * computed from it -- depends on which operators the process factored before.
+ * With it the factor depends only on the operator. */
+#define WUST_FACTOR_FIXED_BUDGET 0x200
 
 typedef struct wust_helmholtz wust_helmholtz;
 

@@ -111,7 +111,8 @@ static void validate() {
     bad.dy = 2 * bad.dx;
     reject(bad, WUST_MODE_REFINED, "LDR9 on rectangular cells");
     reject(g, 7, "unknown mode");
-    reject(g, WUST_MODE_REFINED | 0x200, "unknown mode flag");
+    reject(g, WUST_MODE_REFINED | 0x400, "unknown mode flag");
+    reject(g, WUST_MODE_REFINED | WUST_FACTOR_GAUSS_JORDAN | WUST_FACTOR_FIXED_BUDGET, "Gauss-Jordan with a budget");
     EXPECT(wust_helmholtz_create(&g, nullptr, nullptr, 0, nullptr, nullptr) == WUST_INVALID_INPUT, "null arguments");
 
     // entry points on a null handle

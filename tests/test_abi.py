@@ -68,8 +68,10 @@ def test_argument_validation_without_device():
     desc.stencil_vmin = 1400
     assert lib.wust_helmholtz_create(C.byref(desc), fake, None, 7, None, C.byref(h)) == 1
     assert b"unknown mode" in lib.wust_last_error()
-    assert lib.wust_helmholtz_create(C.byref(desc), fake, None, 0x200, None, C.byref(h)) == 1
+    assert lib.wust_helmholtz_create(C.byref(desc), fake, None, 0x400, None, C.byref(h)) == 1
     assert b"unknown mode flag" in lib.wust_last_error()
+    assert lib.wust_helmholtz_create(C.byref(desc), fake, None, 0x300, None, C.byref(h)) == 1
+    assert b"no Newton-Schulz budget" in lib.wust_last_error()
     assert not h.value
 
 

@@ -74,6 +74,27 @@ def test_learned_budgets_keep_the_factor(ring):
     assert _rel(u2, u1) <= 1e-5
 
 
+def test_fixed_budget_factor_is_history_free(ring):
+    """factor='fixed' (WUST_FACTOR_FIXED_BUDGET): Newton-Schulz with the default round budgets, so
+    the factor depends only on the operator, not on the profile that earlier factorizations of the
+    same shape and frequency left behind: two fixed-budget operators built before and after learned
+    ('auto') factorizations give bit-identical c64 solutions in both directions, and agree with the
+    learned-budget factor to FP32 inverse accuracy.  The sweep kernel variants all accumulate in
+    the same order, so the per-shape variant choice does not enter either."""
+    src = _sources(ring, 8)
+    before = _solver(ring, "c64", "fixed")
+    u_b = [before.solve(src, adj)[0].clone() for adj in (False, True)]
+    before.close()
+    learned = [_solver(ring, "c64", "auto") for _ in range(2)]   # the second runs on a learned profile
+    u_l = learned[1].solve(src, False)[0]
+    after = _solver(ring, "c64", "fixed")
+    u_a = [after.solve(src, adj)[0] for adj in (False, True)]
+    info = after.info()
+    assert info["factor_fallback"] == 0 and info["iterative_blocks"] > 0
+    assert torch.equal(u_a[0], u_b[0]) and torch.equal(u_a[1], u_b[1])
+    assert _rel(u_l, u_a[0]) <= 1e-5
+
+
 def test_newton_schulz_refined_agrees(ring):
     src = _sources(ring, 8)
     u_ns, _ = _solver(ring, "refined", "auto").solve(src, False, tol=1e-13)

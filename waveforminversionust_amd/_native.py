@@ -20,7 +20,8 @@ STATUS_IDS = {
 MODE_REFINED, MODE_C64, MODE_PLAIN = 0, 1, 2
 MODES = {"refined": MODE_REFINED, "c64": MODE_C64, "plain": MODE_PLAIN}
 FACTOR_GAUSS_JORDAN = 0x100
-FACTORS = {"auto": 0, "gauss-jordan": FACTOR_GAUSS_JORDAN}
+FACTOR_FIXED_BUDGET = 0x200
+FACTORS = {"auto": 0, "fixed": FACTOR_FIXED_BUDGET, "gauss-jordan": FACTOR_GAUSS_JORDAN}
 
 
 class WustError(RuntimeError):

@@ -495,7 +495,9 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         const int flags = mode & ~0xff;
         mode &= 0xff;
         require(mode == WUST_MODE_REFINED || mode == WUST_MODE_C64 || mode == WUST_MODE_PLAIN, "unknown mode");
-        require((flags & ~WUST_FACTOR_GAUSS_JORDAN) == 0, "unknown mode flag");
+        require((flags & ~(WUST_FACTOR_GAUSS_JORDAN | WUST_FACTOR_FIXED_BUDGET)) == 0, "unknown mode flag");
+        require(!((flags & WUST_FACTOR_GAUSS_JORDAN) && (flags & WUST_FACTOR_FIXED_BUDGET)),
+                "the Gauss-Jordan factor has no Newton-Schulz budget");
         const wust_grid_desc &g = *desc;
         require(g.ny >= 3 && g.nx >= 3, "Expected a positive finite [ny,nx] medium (min size 3)");
         require(std::isfinite(g.dx) && std::isfinite(g.dy) && g.dx > 0 && g.dy > 0, "Uniform increasing grids required");
@@ -521,6 +523,7 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         op.p = op.m / 2;
         op.mode = mode;
         op.factor_method = (flags & WUST_FACTOR_GAUSS_JORDAN) ? FACTOR_GJ : FACTOR_AUTO;
+        op.fixed_budget = (flags & WUST_FACTOR_FIXED_BUDGET) != 0;
         stencil_params(g.stencil_vmin, g.stencil_vmax, g.frequency_hz, g.dx, g.dy / g.dx, op.b, op.d, op.e);
         AssemblyParams &ap = op.ap;
         ap.ny = g.ny;

@@ -22,6 +22,7 @@ struct Operator {
     int ny = 0, nx = 0, n = 0, m = 0, npad = 0, p = 0;  // p: twisted meeting block
     int mode = 0;
     int factor_method = FACTOR_AUTO;
+    bool fixed_budget = false;    // Newton-Schulz with the default budgets: no learned per-key profile
     double b = 0, d = 0, e = 0;
     AssemblyParams ap{};
     cd *ck2 = nullptr, *vsm = nullptr, *w64 = nullptr;

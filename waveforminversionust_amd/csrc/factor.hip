@@ -876,7 +876,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
         if (c >= 1 && c <= NS_KMAX) run.update_cap = c;
     }
     const char *fixed = std::getenv("WUST_NS_FIXED_BUDGET");
-    const bool learn = !(fixed && fixed[0] == '1');
+    const bool learn = !(fixed && fixed[0] == '1') && !op.fixed_budget;
     const BudgetKey key = budget_key(op);
     if (learn && run.allow_ns && op.ns_rounds) {
         std::lock_guard<std::mutex> lk(g_budget_mu);

@@ -85,10 +85,12 @@ class HelmholtzSolver:
     ``options`` must hold ``backend`` ('gpu') and ``stencil_bounds`` [min, max];
     optional ``dispersion`` = dict(time_step_s, reference_speed_mps) selects
     kwave-ldr9, ``mode`` in {'refined', 'c64', 'plain'} selects the arithmetic
-    (default 'refined'), ``factor`` in {'auto', 'gauss-jordan'} how the engine
-    inverts Schur blocks (default 'auto': Newton-Schulz from the neighbouring
-    inverse with a Gauss-Jordan fallback), and ``tol`` / ``max_iter`` the
-    refinement stop.
+    (default 'refined'), ``factor`` in {'auto', 'fixed', 'gauss-jordan'} how the
+    engine inverts Schur blocks (default 'auto': Newton-Schulz from the neighbouring
+    inverse with a Gauss-Jordan fallback, round budgets learned from the previous
+    factorization of the same shape and frequency; 'fixed': the same with the
+    default budgets, so that the factor depends only on the operator), and
+    ``tol`` / ``max_iter`` the refinement stop.
     """
 
     def __init__(self, x, y, vel, atten, f, signConvention, a0, L_PML, options):
