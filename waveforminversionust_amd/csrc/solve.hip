@@ -313,7 +313,11 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
 // only its own LDS tile and LDS operations of one wave complete in order, so no barrier is needed;
 // DEPTH k-blocks of loads are in flight.  Same K split, accumulation order and epilogue as the
 // 16-row body with KS = 4, so the results are bit-identical to it.
-template <int ALAY, int DEPTH>
+// LDS_B: B staged through LDS too (else each lane loads its B operands directly, 8 bytes per lane:
+// B's rows are contiguous along the MFMA lanes, so those loads already read whole lines).  The
+// epilogue operands are loaded after the GEMM (loading them before it, to hide their latency,
+// measured slower: more registers live across the k loop).
+template <int ALAY, int DEPTH, bool LDS_B = true>
 __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
     constexpr int KS = 4, STR = 17;   // LDS row stride in complex elements (16 + 1 pad: no bank conflicts)
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
@@ -337,9 +341,9 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
     const bool own = k >= o0 && k <= o1;
 
     // operand staging: per wave A[k][i] and B[k][j] of one k-block
-    __shared__ cf ops_lds[4][2][16][STR];
+    __shared__ cf ops_lds[4][LDS_B ? 2 : 1][16][STR];
     cf(*LA)[STR] = ops_lds[wave][0];
-    cf(*LB)[STR] = ops_lds[wave][1];
+    cf(*LB)[STR] = ops_lds[wave][LDS_B ? 1 : 0];   // (unused without LDS_B)
     const int64_t a_off = (ALAY == A_COL) ? i0 : (int64_t)i0 * npad;
     const auto ra = __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(tk.X + a_off), (short)0,
                                                       (int)(((int64_t)npad * npad - a_off) * 8), 0x00020000);
@@ -349,15 +353,23 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
     // is a k of A_COL and B (16 rows / columns), a row of A_CONJT (16 k)
     const int ln = lane >> 3, el = 2 * (lane & 7);
     const uint32_t va = (uint32_t)((ln * npad + el) * 8), vb = (uint32_t)((ln * Rpad + el) * 8);
+    const uint32_t vbd = (uint32_t)((4 * g * Rpad + c) * 8);   // direct B: row 4 g + s, column c
     struct Stage {
-        v4f a[2], b[2];
+        v4f a[2], b[2];   // b: B lines (LDS_B) or, as complex pairs, B[4 g + s][c] for s = 0..3
     };
     auto gload = [&](Stage &r, int k0) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const uint32_t sa = (ALAY == A_COL) ? (uint32_t)((k0 + 8 * q) * npad * 8) : (uint32_t)((8 * q * npad + k0) * 8);
             r.a[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(ra, va, sa, 0));
-            r.b[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rbuf, vb, (uint32_t)((k0 + 8 * q) * Rpad * 8), 0));
+            if (LDS_B) {
+                r.b[q] = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(rbuf, vb, (uint32_t)((k0 + 8 * q) * Rpad * 8), 0));
+            } else {
+                typedef float v2f_t __attribute__((ext_vector_type(2)));
+                const v2f_t u0 = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rbuf, vbd, (uint32_t)((k0 + 2 * q) * Rpad * 8), 0));
+                const v2f_t u1 = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rbuf, vbd, (uint32_t)((k0 + 2 * q + 1) * Rpad * 8), 0));
+                r.b[q] = v4f{u0.x, u0.y, u1.x, u1.y};
+            }
         }
     };
     auto stage = [&](const Stage &r) {
@@ -371,10 +383,34 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
                 LA[el][ln8] = mk(r.a[q].x, -r.a[q].y);
                 LA[el + 1][ln8] = mk(r.a[q].z, -r.a[q].w);
             }
-            LB[ln8][el] = mk(r.b[q].x, r.b[q].y);
-            LB[ln8][el + 1] = mk(r.b[q].z, r.b[q].w);
+            if (LDS_B) {
+                LB[ln8][el] = mk(r.b[q].x, r.b[q].y);
+                LB[ln8][el + 1] = mk(r.b[q].z, r.b[q].w);
+            }
         }
     };
+    // epilogue operands
+    cf pbase[4];
+    cf pn = mk(0.f, 0.f), pc = mk(0.f, 0.f), ps = mk(0.f, 0.f), psrc = mk(0.f, 0.f);
+    auto load_epilogue = [&] {
+        if (tk.base && ks == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) pbase[r] = tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
+        }
+        if (own && tk.Bn) {
+            if (!tk.transposed) {
+                pc = tk.cc[k];
+                if (k >= 1) pn = tk.cn[k];
+                if (k + 1 < npad) ps = tk.cs[k];
+            } else {
+                pc = conjf_(tk.cc[k]);
+                if (k >= 1) pn = conjf_(tk.cs[k - 1]);
+                if (k + 1 < npad) ps = conjf_(tk.cn[k + 1]);
+            }
+            psrc = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + j0 + cc] : mk(0.f, 0.f);
+        }
+    };
+
     using Acc = CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH, SWEEP_FOLD>;
     Acc acc;
     acc.zero();
@@ -400,7 +436,14 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
                         o.a[0][s] = LA[4 * g + s][c];
-                        o.b[0][s] = LB[4 * g + s][c];
+                        if (LDS_B) o.b[0][s] = LB[4 * g + s][c];
+                    }
+                    if (!LDS_B) {
+                        const Stage &r = st[u];
+                        o.b[0][0] = mk(r.b[0].x, r.b[0].y);
+                        o.b[0][1] = mk(r.b[0].z, r.b[0].w);
+                        o.b[0][2] = mk(r.b[1].x, r.b[1].y);
+                        o.b[0][3] = mk(r.b[1].z, r.b[1].w);
                     }
                     acc.mma(o);
                     acc.fold();
@@ -410,25 +453,7 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
         }
     }
 
-    // epilogue operands, loaded after the GEMM (keeps the loop's registers free)
-    cf pbase[4];
-    cf pn = mk(0.f, 0.f), pc = mk(0.f, 0.f), ps = mk(0.f, 0.f), psrc = mk(0.f, 0.f);
-    if (tk.base && ks == 0) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) pbase[r] = tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
-    }
-    if (own && tk.Bn) {
-        if (!tk.transposed) {
-            pc = tk.cc[k];
-            if (k >= 1) pn = tk.cn[k];
-            if (k + 1 < npad) ps = tk.cs[k];
-        } else {
-            pc = conjf_(tk.cc[k]);
-            if (k >= 1) pn = conjf_(tk.cs[k - 1]);
-            if (k + 1 < npad) ps = conjf_(tk.cn[k + 1]);
-        }
-        psrc = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + j0 + cc] : mk(0.f, 0.f);
-    }
+    load_epilogue();
 
     __shared__ float4 red[4][2][64];
     __shared__ cf tile[16][17];
@@ -477,9 +502,9 @@ template <int ALAY, int DEPTH, bool LATE, bool BUF>
 __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
     sweep_fused_body<ALAY, DEPTH, LATE, BUF>(p);
 }
-template <int ALAY, int DEPTH>
+template <int ALAY, int DEPTH, bool LDS_B>
 __global__ void __launch_bounds__(256) sweep_fused_kernel_lds(FusedParams p) {
-    sweep_fused_body_lds<ALAY, DEPTH>(p);
+    sweep_fused_body_lds<ALAY, DEPTH, LDS_B>(p);
 }
 
 // Zeroes rows r0 .. npad-1 of the [npad][Rpad] blocks of two arrays (blockIdx.y selects the
@@ -499,14 +524,15 @@ __global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r
 // Kernel variants of the fused step and their per-shape choice.  All take 16-row tiles with K dealt
 // over the four waves of a workgroup in the same order, so they give bit-identical results; they
 // differ in how the operands reach the MFMA:
-//   0, 1  16-byte loads of whole lines staged through a wave-private LDS tile, DEPTH 1 / 2
-//   2     buffer loads into registers, epilogue operands loaded after the GEMM, DEPTH 1
-//   3     buffer loads into registers, DEPTH 2
-//   4     flat loads into registers, DEPTH 2
+//   0, 2  A and B fetched as whole lines, 16 bytes per lane, staged through a wave-private LDS tile
+//         (DEPTH 1 / 2)
+//   1     A the same way, B loaded directly into the MFMA lanes (its rows are contiguous along them)
+//   3     buffer loads of both straight into the MFMA lanes, epilogue operands loaded after the GEMM
+//   4     flat loads of both, DEPTH 2 (the reference variant)
 constexpr int N_FUSED_VARIANTS = 5;
-constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {1, 2, 1, 2, 2};
-const char *const FUSED_KIND[N_FUSED_VARIANTS] = {"LDS-staged", "LDS-staged", "buffer loads, late epilogue operands",
-                                                  "buffer loads", "flat loads"};
+constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {1, 1, 2, 1, 2};
+const char *const FUSED_KIND[N_FUSED_VARIANTS] = {"A and B staged through LDS", "A staged through LDS, B direct",
+                                                  "A and B staged through LDS", "buffer loads", "flat loads"};
 
 // strips of 16-row tiles over n rows (each owns FUSED_OWN rows), and the rows they cover
 inline int fused_strips(int n) { return (n + FUSED_OWN - 1) / FUSED_OWN; }
@@ -530,10 +556,10 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
         }
     };
     switch (v) {
-        case 0: go(sweep_fused_kernel_lds<ALAY, 1>); break;
-        case 1: go(sweep_fused_kernel_lds<ALAY, 2>); break;
-        case 2: go(sweep_fused_kernel<ALAY, 1, true, true>); break;
-        case 3: go(sweep_fused_kernel<ALAY, 2, false, true>); break;
+        case 0: go(sweep_fused_kernel_lds<ALAY, 1, true>); break;
+        case 1: go(sweep_fused_kernel_lds<ALAY, 1, false>); break;
+        case 2: go(sweep_fused_kernel_lds<ALAY, 2, true>); break;
+        case 3: go(sweep_fused_kernel<ALAY, 1, true, true>); break;
         default: go(sweep_fused_kernel<ALAY, 2, false, false>); break;
     }
 }
