@@ -315,9 +315,10 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
 // 16-row body with KS = 4, so the results are bit-identical to it.
 // LDS_B: B staged through LDS too (else each lane loads its B operands directly, 8 bytes per lane:
 // B's rows are contiguous along the MFMA lanes, so those loads already read whole lines).  The
-// epilogue operands are loaded after the GEMM (loading them before it, to hide their latency,
-// measured slower: more registers live across the k loop).
-template <int ALAY, int DEPTH, bool LDS_B = true>
+// epilogue operands are loaded after the GEMM, or with MID right after the first k-block's operand
+// loads, so that the GEMM hides their latency without their loads delaying its first MFMAs
+// (loading them before the GEMM's first loads measured slower).
+template <int ALAY, int DEPTH, bool LDS_B = true, bool MID = false>
 __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
     constexpr int KS = 4, STR = 17;   // LDS row stride in complex elements (16 + 1 pad: no bank conflicts)
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
@@ -425,6 +426,11 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
         Stage st[DEPTH + 1];
 #pragma unroll
         for (int u = 0; u < DEPTH; ++u) gload(st[u], k0(u));
+        if (MID) {
+            __builtin_amdgcn_sched_barrier(0);
+            load_epilogue();
+            __builtin_amdgcn_sched_barrier(0);
+        }
         for (int t0 = 0; t0 < nk; t0 += DEPTH + 1) {
 #pragma unroll
             for (int u = 0; u <= DEPTH; ++u) {
@@ -453,7 +459,7 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
         }
     }
 
-    load_epilogue();
+    if (!MID) load_epilogue();
 
     __shared__ float4 red[4][2][64];
     __shared__ cf tile[16][17];
@@ -502,9 +508,9 @@ template <int ALAY, int DEPTH, bool LATE, bool BUF>
 __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
     sweep_fused_body<ALAY, DEPTH, LATE, BUF>(p);
 }
-template <int ALAY, int DEPTH, bool LDS_B>
+template <int ALAY, int DEPTH, bool LDS_B, bool MID = false>
 __global__ void __launch_bounds__(256) sweep_fused_kernel_lds(FusedParams p) {
-    sweep_fused_body_lds<ALAY, DEPTH, LDS_B>(p);
+    sweep_fused_body_lds<ALAY, DEPTH, LDS_B, MID>(p);
 }
 
 // Zeroes rows r0 .. npad-1 of the [npad][Rpad] blocks of two arrays (blockIdx.y selects the
@@ -529,10 +535,14 @@ __global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r
 //   1     A the same way, B loaded directly into the MFMA lanes (its rows are contiguous along them)
 //   3     buffer loads of both straight into the MFMA lanes, epilogue operands loaded after the GEMM
 //   4     flat loads of both, DEPTH 2 (the reference variant)
-constexpr int N_FUSED_VARIANTS = 5;
-constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {1, 1, 2, 1, 2};
+//   5, 6  variants 0 and 1 with the epilogue operands loaded right after the first k-block's operands
+//         (their latency hidden by the GEMM, without delaying its first MFMAs)
+constexpr int N_FUSED_VARIANTS = 7;
+constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {1, 1, 2, 1, 2, 1, 1};
 const char *const FUSED_KIND[N_FUSED_VARIANTS] = {"A and B staged through LDS", "A staged through LDS, B direct",
-                                                  "A and B staged through LDS", "buffer loads", "flat loads"};
+                                                  "A and B staged through LDS", "buffer loads", "flat loads",
+                                                  "A and B staged through LDS, epilogue operands early",
+                                                  "A staged through LDS, B direct, epilogue operands early"};
 
 // strips of 16-row tiles over n rows (each owns FUSED_OWN rows), and the rows they cover
 inline int fused_strips(int n) { return (n + FUSED_OWN - 1) / FUSED_OWN; }
@@ -560,6 +570,8 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
         case 1: go(sweep_fused_kernel_lds<ALAY, 1, false>); break;
         case 2: go(sweep_fused_kernel_lds<ALAY, 2, true>); break;
         case 3: go(sweep_fused_kernel<ALAY, 1, true, true>); break;
+        case 5: go(sweep_fused_kernel_lds<ALAY, 1, true, true>); break;
+        case 6: go(sweep_fused_kernel_lds<ALAY, 1, false, true>); break;
         default: go(sweep_fused_kernel<ALAY, 2, false, false>); break;
     }
 }
