@@ -520,6 +520,8 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         op.n = g.ny - 2;
         op.m = g.nx - 2;
         op.npad = (int)round_up(op.n, 32);
+        // the factor's GEMMs address an inverse page with 32-bit byte offsets (npad^2 x 8 bytes)
+        require((int64_t)op.npad * op.npad * 8 <= INT32_MAX, "grid too large: ny - 2 must not exceed 16352");
         op.p = op.m / 2;
         op.mode = mode;
         op.factor_method = (flags & WUST_FACTOR_GAUSS_JORDAN) ? FACTOR_GJ : FACTOR_AUTO;

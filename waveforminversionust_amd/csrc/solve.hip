@@ -585,7 +585,13 @@ struct FusedKey {
 std::mutex g_fused_mu;
 std::map<FusedKey, int> g_fused_choice;
 
-bool fused_eligible(int v, const FusedParams &fp) { return v >= 0 && v < N_FUSED_VARIANTS && fp.npad >= 16; }
+// Buffer-resource variants address a page and an operand block with 32-bit byte offsets (and a 32-bit
+// extent); larger blocks take the flat-load variant.
+bool fused_eligible(int v, const FusedParams &fp) {
+    if (v < 0 || v >= N_FUSED_VARIANTS || fp.npad < 16) return false;
+    const int64_t page_bytes = (int64_t)fp.npad * fp.npad * 8, block_bytes = (int64_t)fp.npad * fp.Rpad * 8;
+    return v == 4 || (page_bytes <= INT32_MAX && block_bytes <= INT32_MAX);
+}
 
 // Times every eligible variant on `fp` (a real step whose outputs the sweep rewrites before
 // reading them) and returns the fastest; the choice is kept per (npad, Rpad, adjoint).  Within a
@@ -602,7 +608,7 @@ int fused_variant(const FusedParams &fp, bool adjoint, hipStream_t s, const Oper
         const char *e = std::getenv("WUST_AUTOTUNE");
         return !(e && e[0] == '0');
     }();
-    if (!tune) return 0;
+    if (!tune) return fused_eligible(0, fp) ? 0 : 4;
     const FusedKey key{fp.npad, fp.Rpad, adjoint ? 1 : 0};
     {
         std::lock_guard<std::mutex> lk(g_fused_mu);
