@@ -11,8 +11,9 @@
 // or three (M3 = true, the 3M form):
 //   p1 = ar br,  p2 = ai bi,  p3 = (ar + ai)(br + bi);   re = p1 - p2,  im = p3 - p1 - p2
 // The 3M form saves a quarter of the MFMA work; its imaginary part carries
-// the rounding of |a||b|-sized terms, so it serves self-correcting iterations
-// (Newton-Schulz) but not the sweeps, whose accuracy sets the refinement count.
+// the rounding of |ar + ai||br + bi|-sized terms.  Over a whole reduction in
+// FP32 that costs the block solves a refinement correction; with FOLD (below)
+// the sweeps' contraction stays where the four-product form has it.
 //
 // Operand layouts (lda / ldb in complex elements):
 //   A_COL    A[i][k] = P[k lda + i]          column-major (inverse pages, Schur blocks)
@@ -24,9 +25,9 @@
 // sets rotated by unrolling, no register copies), so a wave waits only for
 // data requested DEPTH k-blocks earlier.
 //
-// FOLD = true (four-real-product form only): the MFMA accumulators hold one
-// k-block (16 products per element) at a time and are added into FP64 running
-// sums after every k-block.  A long FP32 accumulation over the whole reduction
+// FOLD = true: the MFMA accumulators hold one k-block (16 products per
+// element) at a time and are added into FP64 running sums after every k-block
+// (the 3M combination re = p1 - p2, im = p3 - (p1 + p2) formed in FP64).  A long FP32 accumulation over the whole reduction
 // loses accuracy in proportion to its length; the block solves' contraction in
 // iterative refinement is set by exactly that rounding, so the sweeps fold.
 #pragma once
@@ -41,7 +42,6 @@ enum { B_KCOL = 0, B_ROW = 1 };
 
 template <int RB, int CB, int ALAY, int BLAY, bool M3, int DEPTH = 2, bool FOLD = false>
 struct CgemmWave {
-    static_assert(!(FOLD && M3), "the FP64 fold serves the four-real-product form");
     // operands of one k-block: a[rb][s], b[cb][s]
     struct Ops {
         cf a[RB][4], b[CB][4];
@@ -73,10 +73,16 @@ struct CgemmWave {
                 for (int y = 0; y < CB; ++y) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        q1[x][y][r] += (double)p1[x][y][r];
-                        q3[x][y][r] += (double)p3[x][y][r];
+                        if (M3) {
+                            const double a = p1[x][y][r], b = p2[x][y][r], d = p3[x][y][r];
+                            q1[x][y][r] += a - b;
+                            q3[x][y][r] += d - (a + b);
+                        } else {
+                            q1[x][y][r] += (double)p1[x][y][r];
+                            q3[x][y][r] += (double)p3[x][y][r];
+                        }
                     }
-                    p1[x][y] = p3[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
+                    p1[x][y] = p2[x][y] = p3[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
                 }
         }
     }
@@ -286,12 +292,12 @@ struct CgemmWave {
     __device__ __forceinline__ void result(int rb, int cb, float (&re)[4], float (&im)[4]) const {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            if (M3) {
-                re[r] = p1[rb][cb][r] - p2[rb][cb][r];
-                im[r] = p3[rb][cb][r] - p1[rb][cb][r] - p2[rb][cb][r];
-            } else if (FOLD) {
+            if (FOLD) {
                 re[r] = (float)q1[rb][cb][r];
                 im[r] = (float)q3[rb][cb][r];
+            } else if (M3) {
+                re[r] = p1[rb][cb][r] - p2[rb][cb][r];
+                im[r] = p3[rb][cb][r] - p1[rb][cb][r] - p2[rb][cb][r];
             } else {
                 re[r] = p1[rb][cb][r];
                 im[r] = p3[rb][cb][r];

@@ -4,8 +4,9 @@
 // tridiagonal kernel and one cuBLAS Cgemm3m (Ny x R x Ny), serially over x
 // for the forward and backward passes.  Here each step is an operand pass
 // (the tridiagonal couplings, evaluated once per element) followed by a
-// complex GEMM on MFMA (v_mfma_f32_16x16x4f32, four real products per complex
-// product, FP32 accumulation), and the twisted factorization lets the left and
+// complex GEMM on MFMA (v_mfma_f32_16x16x4f32, three real products per complex
+// product, FP32 accumulation per k-block of 16 folded into FP64 sums), and the
+// twisted factorization lets the left and
 // right chains of a step run in the same launches:
 //   forward  left  y_j = X_j (b_j - L_j y_{j-1}),       j = 0..p-1
 //            right z_j = X_j (b_j - U_j z_{j+1}),       j = m-1..p+1
@@ -29,6 +30,9 @@
 #ifndef WUST_SWEEP_FOLD
 #define WUST_SWEEP_FOLD 1
 #endif
+#ifndef WUST_SWEEP_M3
+#define WUST_SWEEP_M3 1
+#endif
 
 namespace wust {
 
@@ -36,7 +40,11 @@ namespace {
 
 // The sweep products accumulate each k-block on MFMA and fold it into FP64 sums (CgemmWave FOLD): a
 // long FP32 accumulation over the block rows sets the refinement contraction of the whole solve.
+// They take the 3M form (three real products per complex product): with the fold its extra
+// rounding leaves the contraction where four products have it (at 560^2 x 256 RHS the relative
+// residual after the first c64 solve is 3.3e-6 forward / 5.4e-6 adjoint, against 3.7e-6 / 5.1e-6).
 constexpr bool SWEEP_FOLD = WUST_SWEEP_FOLD != 0;
+constexpr bool SWEEP_M3 = WUST_SWEEP_M3 != 0;
 
 struct SweepTerm {
     const cf *cn, *cc, *cs;   // tridiagonal row coefficients (block of the coupling)
@@ -94,7 +102,7 @@ __global__ void __launch_bounds__(256) sweep_operand_kernel(SweepParams p) {
 
 // GEMM pass: out (+)= op(X) B, op(X) = X or X^H, on tiles of SM x 16 (SM = 16 RB
 // block rows x 16 RHS).  The 4 waves of a WG deal the k-blocks of 16
-// round-robin and each accumulates the whole tile (CgemmWave: four real
+// round-robin and each accumulates the whole tile (CgemmWave: three real
 // products per complex product, two k-blocks of operands in flight, straight
 // from global memory in the MFMA lane layout); the partial tiles are summed
 // through LDS.  Tiles
@@ -115,7 +123,7 @@ __global__ void __launch_bounds__(256) sweep_gemm_kernel(SweepParams p) {
     const int i0 = mb * 16 * RB, j0 = nb * 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, c = lane & 15;
 
-    CgemmWave<RB, 1, ALAY, B_ROW, false, 2, SWEEP_FOLD> acc;
+    CgemmWave<RB, 1, ALAY, B_ROW, SWEEP_M3, 2, SWEEP_FOLD> acc;
     acc.zero();
     const cf *A = (ALAY == A_COL) ? tk.X + i0 : tk.X + (int64_t)i0 * npad;
     acc.run(A, npad, tk.B + j0, Rpad, wave, npad / 16, 4, g, c);
@@ -241,7 +249,7 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
     };
     if (!LATE) load_epilogue();
 
-    CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH, FOLD> acc;
+    CgemmWave<1, 1, ALAY, B_ROW, SWEEP_M3, DEPTH, FOLD> acc;
     acc.zero();
     const int64_t a_off = (ALAY == A_COL) ? i0 : (int64_t)i0 * npad;
     const cf *A = tk.X + a_off;
@@ -412,7 +420,7 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
         }
     };
 
-    using Acc = CgemmWave<1, 1, ALAY, B_ROW, false, DEPTH, SWEEP_FOLD>;
+    using Acc = CgemmWave<1, 1, ALAY, B_ROW, SWEEP_M3, DEPTH, SWEEP_FOLD>;
     Acc acc;
     acc.zero();
     {
@@ -451,7 +459,12 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
                         o.b[0][2] = mk(r.b[1].x, r.b[1].y);
                         o.b[0][3] = mk(r.b[1].z, r.b[1].w);
                     }
+                    // raised wave priority for the MFMA phase: a wave that has its operands issues
+                    // its MFMAs ahead of the other waves' staging work, so the waves of a SIMD
+                    // drift apart in phase instead of all waiting for the matrix core together
+                    __builtin_amdgcn_s_setprio(1);
                     acc.mma(o);
+                    __builtin_amdgcn_s_setprio(0);
                     acc.fold();
                 }
                 __builtin_amdgcn_sched_barrier(0);
