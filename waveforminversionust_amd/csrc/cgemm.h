@@ -27,7 +27,7 @@
 //
 // FOLD = true: the MFMA accumulators hold one k-block (16 products per
 // element) at a time and are added into FP64 running sums after every k-block
-// (the 3M combination re = p1 - p2, im = p3 - (p1 + p2) formed in FP64).  A long FP32 accumulation over the whole reduction
+// (the 3M combination re = p1 - p2, im = p3 - (p1 + p2) formed per k-block).  A long FP32 accumulation over the whole reduction
 // loses accuracy in proportion to its length; the block solves' contraction in
 // iterative refinement is set by exactly that rounding, so the sweeps fold.
 #pragma once
@@ -74,9 +74,11 @@ struct CgemmWave {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         if (M3) {
-                            const double a = p1[x][y][r], b = p2[x][y][r], d = p3[x][y][r];
-                            q1[x][y][r] += a - b;
-                            q3[x][y][r] += d - (a + b);
+                            // the 3M combination in FP32 (its rounding is |p|-sized, below the 3M
+                            // products' own), then into the FP64 sums
+                            const float a = p1[x][y][r], b = p2[x][y][r], d = p3[x][y][r];
+                            q1[x][y][r] += (double)(a - b);
+                            q3[x][y][r] += (double)(d - (a + b));
                         } else {
                             q1[x][y][r] += (double)p1[x][y][r];
                             q3[x][y][r] += (double)p3[x][y][r];
@@ -163,7 +165,10 @@ struct CgemmWave {
     // before each MFMA block are static; only the MFMA blocks are guarded, by
     // wave-uniform branches.  sched_barrier(0) keeps the scheduler from moving
     // loads across the MFMA phases, which would make the wait counts cover
-    // later loads.
+    // later loads.  The MFMA phase runs at raised wave priority: a wave whose
+    // operands have arrived issues its MFMAs ahead of the other waves' loads
+    // and folds, so the waves of a SIMD drift apart in phase and keep the
+    // matrix core busy instead of all waiting for it at once.
     __device__ __forceinline__ void run(const cf *A, int64_t lda, const cf *B, int64_t ldb, int kb0, int kb_end,
                                         int step, int g, int c) {
         const int nk = (kb_end - kb0 + step - 1) / step;
@@ -182,7 +187,9 @@ struct CgemmWave {
                 load(o[(u + DEPTH) % (DEPTH + 1)], A, lda, B, ldb, k0(t + u + DEPTH), g, c);
                 __builtin_amdgcn_sched_barrier(0);
                 if (t + u < nk) {
+                    __builtin_amdgcn_s_setprio(1);
                     mma(o[u]);
+                    __builtin_amdgcn_s_setprio(0);
                     fold();
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -280,7 +287,9 @@ struct CgemmWave {
                 load_buf(o[(u + DEPTH) % (DEPTH + 1)], q, k0(t + u + DEPTH));
                 __builtin_amdgcn_sched_barrier(0);
                 if (t + u < nk) {
+                    __builtin_amdgcn_s_setprio(1);   // MFMA phase first (see run())
                     mma(o[u]);
+                    __builtin_amdgcn_s_setprio(0);
                     fold();
                 }
                 __builtin_amdgcn_sched_barrier(0);
