@@ -12,8 +12,8 @@
 //   p1 = ar br,  p2 = ai bi,  p3 = (ar + ai)(br + bi);   re = p1 - p2,  im = p3 - p1 - p2
 // The 3M form saves a quarter of the MFMA work; its imaginary part carries
 // the rounding of |ar + ai||br + bi|-sized terms.  Over a whole reduction in
-// FP32 that costs the block solves a refinement correction; with FOLD (below)
-// the sweeps' contraction stays where the four-product form has it.
+// one FP32 accumulator that costs the block solves a refinement correction;
+// with FOLD (below) the contraction stays where the four-product form has it.
 //
 // Operand layouts (lda / ldb in complex elements):
 //   A_COL    A[i][k] = P[k lda + i]          column-major (inverse pages, Schur blocks)
@@ -26,10 +26,15 @@
 // data requested DEPTH k-blocks earlier.
 //
 // FOLD = true: the MFMA accumulators hold one k-block (16 products per
-// element) at a time and are added into FP64 running sums after every k-block
-// (the 3M combination re = p1 - p2, im = p3 - (p1 + p2) formed per k-block).  A long FP32 accumulation over the whole reduction
-// loses accuracy in proportion to its length; the block solves' contraction in
-// iterative refinement is set by exactly that rounding, so the sweeps fold.
+// element) at a time; after every k-block its complex result (the 3M
+// combination re = p1 - p2, im = p3 - (p1 + p2) formed per k-block) is added
+// into separate FP32 running sums.  One FP32 accumulation over the whole
+// reduction (576 products at 560^2) loses accuracy in proportion to its
+// length, and the block solves' contraction in iterative refinement is set by
+// exactly that rounding; the two-level sum (16 products on MFMA, then the
+// k-block partials) keeps it short.  FP64 running sums measured the same
+// contraction (3.7e-6 forward, 5.8e-6 adjoint at 560^2 x 256 RHS) at more
+// VALU work and registers.
 #pragma once
 #include "wust_common.h"
 
@@ -47,7 +52,7 @@ struct CgemmWave {
         cf a[RB][4], b[CB][4];
     };
     v4f p1[RB][CB], p2[RB][CB], p3[RB][CB];   // p2 unused without M3
-    double q1[FOLD ? RB : 1][FOLD ? CB : 1][4], q3[FOLD ? RB : 1][FOLD ? CB : 1][4];   // FP64 sums (FOLD)
+    float q1[FOLD ? RB : 1][FOLD ? CB : 1][4], q3[FOLD ? RB : 1][FOLD ? CB : 1][4];   // running sums (FOLD)
 
     __device__ __forceinline__ void zero() {
 #pragma unroll
@@ -60,11 +65,11 @@ struct CgemmWave {
 #pragma unroll
                 for (int y = 0; y < CB; ++y)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) q1[x][y][r] = q3[x][y][r] = 0.0;
+                    for (int r = 0; r < 4; ++r) q1[x][y][r] = q3[x][y][r] = 0.f;
         }
     }
 
-    // FOLD: add the k-block held by the MFMA accumulators into the FP64 sums and clear them
+    // FOLD: add the k-block held by the MFMA accumulators into the running sums and clear them
     __device__ __forceinline__ void fold() {
         if constexpr (FOLD) {
 #pragma unroll
@@ -74,14 +79,13 @@ struct CgemmWave {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         if (M3) {
-                            // the 3M combination in FP32 (its rounding is |p|-sized, below the 3M
-                            // products' own), then into the FP64 sums
+                            // the 3M combination of this k-block, then into the running sums
                             const float a = p1[x][y][r], b = p2[x][y][r], d = p3[x][y][r];
-                            q1[x][y][r] += (double)(a - b);
-                            q3[x][y][r] += (double)(d - (a + b));
+                            q1[x][y][r] += a - b;
+                            q3[x][y][r] += d - (a + b);
                         } else {
-                            q1[x][y][r] += (double)p1[x][y][r];
-                            q3[x][y][r] += (double)p3[x][y][r];
+                            q1[x][y][r] += p1[x][y][r];
+                            q3[x][y][r] += p3[x][y][r];
                         }
                     }
                     p1[x][y] = p2[x][y] = p3[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
@@ -302,8 +306,8 @@ struct CgemmWave {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             if (FOLD) {
-                re[r] = (float)q1[rb][cb][r];
-                im[r] = (float)q3[rb][cb][r];
+                re[r] = q1[rb][cb][r];
+                im[r] = q3[rb][cb][r];
             } else if (M3) {
                 re[r] = p1[rb][cb][r] - p2[rb][cb][r];
                 im[r] = p3[rb][cb][r] - p1[rb][cb][r] - p2[rb][cb][r];

@@ -443,8 +443,9 @@ __global__ void __launch_bounds__(256) gj_panel_kernel(GjParams p) {
 // whatever the norm.  Each round is three launches -- residual, cubic
 // operand, update -- and each launch decides on the device, from the norms
 // of earlier launches, whether it has work; launches after a block's last
-// update return at once.  The residual sets the accuracy floor and uses four
-// real products per complex product; the products with E_r (small) use 3M.
+// update return at once.  Every product takes the 3M form (three real
+// products per complex product); the residual, which sets the accuracy floor,
+// sums its k-blocks in two levels (CgemmWave FOLD).
 // ---------------------------------------------------------------------------
 enum NsKind { NS_RESIDUAL = 0, NS_CUBIC = 1, NS_UPDATE = 2 };
 
@@ -487,8 +488,9 @@ __device__ __forceinline__ void ns_tile(int r, int T, int &ti, int &tj) {
 // One 32x32 output tile of  out = C0 + sgn A B  per workgroup (C0 = I when
 // null); the four waves split the reduction (k-blocks of 16 dealt
 // round-robin) and each accumulates the whole tile as 2x2 blocks of 16x16
-// (CgemmWave); the partial tiles are summed through LDS.
-template <bool M3>
+// (CgemmWave); the partial tiles are summed through LDS.  UPDATE: the cubic
+// operand and update launches (else the residual launch).
+template <bool UPDATE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) ns_gemm_kernel(NsParams p) {
     const int T = p.npad / NB, per_task = T * T, G = p.ntasks * per_task;
     const int per_xcd = (G + 7) / 8;
@@ -524,11 +526,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) n
     const int ld = p.npad, i0 = ti * NB, j0 = tj * NB;
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform (scalar k offsets)
-    // the residual E = I - T X (four real products) folds each k-block into FP64 sums: its accuracy
-    // sets the accuracy of the inverse that the update X (I + E) produces
-    // (three workgroups per CU hold a 576^2 product's 648 tiles in one round: the FP64 sums of the
-    // four-product form leave room for one k-block of operands in flight, the 3M form for two)
-    CgemmWave<2, 2, A_COL, B_KCOL, M3, (M3 ? 2 : 1), !M3> acc;
+    // every product takes the 3M form; the residual E = I - T X folds each k-block into running sums
+    // (its accuracy sets the accuracy of the inverse that the update X (I + E) produces), with one
+    // k-block of operands in flight so that three workgroups per CU hold a 576^2 product's 648 tiles
+    // in one round; the updates, without the sums, keep two in flight
+    CgemmWave<2, 2, A_COL, B_KCOL, true, (UPDATE ? 2 : 1), !UPDATE> acc;
     acc.zero();
     const int64_t nn = (int64_t)ld * ld;
     acc.run_buf(A + i0, ld, nn - i0, B + (int64_t)j0 * ld, ld, nn - (int64_t)j0 * ld, wave, ld / 16, 4, g, c);

@@ -5,7 +5,7 @@
 // for the forward and backward passes.  Here each step is an operand pass
 // (the tridiagonal couplings, evaluated once per element) followed by a
 // complex GEMM on MFMA (v_mfma_f32_16x16x4f32, three real products per complex
-// product, FP32 accumulation per k-block of 16 folded into FP64 sums), and the
+// product, FP32 accumulation per k-block of 16 folded into running sums), and the
 // twisted factorization lets the left and
 // right chains of a step run in the same launches:
 //   forward  left  y_j = X_j (b_j - L_j y_{j-1}),       j = 0..p-1
@@ -38,11 +38,11 @@ namespace wust {
 
 namespace {
 
-// The sweep products accumulate each k-block on MFMA and fold it into FP64 sums (CgemmWave FOLD): a
-// long FP32 accumulation over the block rows sets the refinement contraction of the whole solve.
-// They take the 3M form (three real products per complex product): with the fold its extra
+// The sweep products accumulate each k-block on MFMA and fold it into running sums (CgemmWave FOLD):
+// one long FP32 accumulation over the block rows would set the refinement contraction of the whole
+// solve.  They take the 3M form (three real products per complex product): with the fold its extra
 // rounding leaves the contraction where four products have it (at 560^2 x 256 RHS the relative
-// residual after the first c64 solve is 3.3e-6 forward / 5.4e-6 adjoint, against 3.7e-6 / 5.1e-6).
+// residual after the first c64 solve is 3.7e-6 forward / 5.9e-6 adjoint).
 constexpr bool SWEEP_FOLD = WUST_SWEEP_FOLD != 0;
 constexpr bool SWEEP_M3 = WUST_SWEEP_M3 != 0;
 
