@@ -53,7 +53,9 @@ def test_newton_schulz_matches_gauss_jordan(ring, adjoint):
     i_ns, i_gj = s_ns.info(), s_gj.info()
     m = ring.nx - 2
     assert (m - 3) // 2 <= i_ns["iterative_blocks"] <= m - 5 and i_ns["factor_fallback"] == 0
-    assert 0 < i_ns["inverse_residual"] <= 1e-5
+    # the reported bound: ||E||_F^2 of a quadratic last update (taken up to 2.5e-5) or ||E||_F^3 of a
+    # cubic one; the solutions below check the inverses themselves
+    assert 0 < i_ns["inverse_residual"] <= 2.5e-5
     assert i_gj["iterative_blocks"] == 0 and i_gj["factor_fallback"] == 0
     u_ns, _ = s_ns.solve(src, adjoint)
     u_gj, _ = s_gj.solve(src, adjoint)
@@ -67,7 +69,7 @@ def test_learned_budgets_keep_the_factor(ring):
     first = _solver(ring, "c64", "auto")
     second = _solver(ring, "c64", "auto")
     i1, i2 = first.info(), second.info()
-    assert i2["factor_fallback"] == 0 and 0 < i2["inverse_residual"] <= 1e-5
+    assert i2["factor_fallback"] == 0 and 0 < i2["inverse_residual"] <= 2.5e-5
     assert i2["iterative_blocks"] <= i1["iterative_blocks"]
     u1, _ = first.solve(src, False)
     u2, _ = second.solve(src, False)
@@ -140,7 +142,7 @@ def test_unconverged_factor_keeps_its_profile(ring, monkeypatch):
         if infos[-1]["factor_fallback"] == 0:
             break
     assert infos[-1]["factor_fallback"] == 0, [i["factor_fallback"] for i in infos]
-    assert infos[-1]["inverse_residual"] <= 1e-5   # (under this cap every block may end on Gauss-Jordan)
+    assert infos[-1]["inverse_residual"] <= 2.5e-5   # (under this cap every block may end on Gauss-Jordan)
     monkeypatch.delenv("WUST_NS_MAX_UPDATES")
     s_gj = G.HelmholtzSolver(ring.x_m, ring.y_m, ring.c_true, None, f, -1, ring.cfg["pml_strength"],
                              ring.cfg["pml_m"], dict(OPTS, mode="c64", factor="gauss-jordan"))
