@@ -137,8 +137,10 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
 int wust_helmholtz_destroy(wust_helmholtz *h);
 
 /* Device memory of destroyed operators is kept (up to WUST_POOL_MB megabytes, default 98304)
- * for the next operator with the same sizes; wust_pool_trim frees all of it.  No reference
- * counterpart (the MEX gateways allocate and free per call). */
+ * for the next operator with the same sizes, and the inverse pages of the leading (PML) chain
+ * steps of the last factorization per operator key (up to WUST_SEED_CACHE_MB, default 4096) seed
+ * the next factorization with that key; wust_pool_trim frees both.  No reference counterpart (the
+ * MEX gateways allocate and free per call). */
 int wust_pool_trim(void);
 int wust_helmholtz_info(const wust_helmholtz *h, wust_info *info);
 

@@ -82,8 +82,12 @@ def test_backward_scales_and_places_gradient(cfg1):
     (3.0 * loss).backward()
     _, g = G.wust_oracle(1 / s.detach().double(), obs, 0, case.cfg)
     assert s.grad.device.type == "cpu" and s.grad.dtype == torch.float32
-    # far from the array the slowness gradient falls below FLT_MIN (subnormal in FP32): compare at that floor
-    assert torch.allclose(s.grad.double(), 3.0 * g.cpu().double(), rtol=1e-6, atol=torch.finfo(torch.float32).tiny)
+    # far from the array the slowness gradient falls below FLT_MIN (subnormal in FP32): compare at that floor,
+    # and the two refined evaluations (the second factorization seeded from the first) agree to the
+    # refinement tolerance relative to the largest entry
+    ref = 3.0 * g.cpu().double()
+    atol = max(torch.finfo(torch.float32).tiny, 1e-9 * float(ref.abs().max()))
+    assert torch.allclose(s.grad.double(), ref, rtol=1e-6, atol=atol)
     with pytest.raises(G.WustError):
         G.wust_loss(-s.detach(), obs, 0, case.cfg)
 

@@ -5,11 +5,13 @@ iteration (factor='auto') except where Gauss-Jordan is cheaper (the second
 column of each chain, and columns that needed 4 or more rounds, 3 or more from
 npad 800 on, in an earlier factorization of the same operator key); the chain starts, the meeting block,
 and the whole factor when an iteration misses its residual bound use
-Gauss-Jordan elimination (factor='gauss-jordan' forces it everywhere).
+Gauss-Jordan elimination (factor='gauss-jordan' forces it everywhere).  A later
+factorization with the same key seeds Newton-Schulz for the leading chain steps
+with the previous factorization's pages of the same blocks instead.
 Tolerances:
 * the two methods give complex-FP32 solutions that agree to <= 1e-5
   relative (both are FP32 inverses with residuals near 1e-6);
-* the Newton-Schulz residual bound reported by the factor is <= 1e-5;
+* the Newton-Schulz residual bound reported by the factor is <= 2.5e-5;
 * refined solves from either factor agree with each other to <= 1e-12;
 * a forced fallback reproduces the Gauss-Jordan factor bit for bit.
 """
@@ -52,7 +54,8 @@ def test_newton_schulz_matches_gauss_jordan(ring, adjoint):
     s_gj = _solver(ring, "c64", "gauss-jordan")
     i_ns, i_gj = s_ns.info(), s_gj.info()
     m = ring.nx - 2
-    assert (m - 3) // 2 <= i_ns["iterative_blocks"] <= m - 5 and i_ns["factor_fallback"] == 0
+    # (up to m - 1 blocks: the leading steps are seeded when an earlier test factored this key)
+    assert (m - 3) // 2 <= i_ns["iterative_blocks"] <= m - 1 and i_ns["factor_fallback"] == 0
     # the reported bound: ||E||_F^2 of a quadratic last update (taken up to 2.5e-5) or ||E||_F^3 of a
     # cubic one; the solutions below check the inverses themselves
     assert 0 < i_ns["inverse_residual"] <= 2.5e-5
@@ -151,3 +154,43 @@ def test_unconverged_factor_keeps_its_profile(ring, monkeypatch):
     u_gj, _ = s_gj.solve(src, False)
     assert _rel(u, u_gj) <= 1e-5
 
+
+
+def test_seeded_leading_steps(ring):
+    """The leading (PML) chain steps of a factorization are seeded from the previous factorization with
+    the same key: after a model change that leaves the PML nearly untouched (a smooth bump), the
+    seeded factor inverts every block but the meeting one by Newton-Schulz without a fallback, and its
+    solutions match a history-free factor of the same model (c64 to FP32 inverse accuracy, refined to
+    1e-12); a large change (the whole model 60 m/s faster) still gives refined solutions to 1e-12
+    (through the fallback where a seed no longer converges); wust_pool_trim drops the seeds."""
+    from waveforminversionust_amd import _native as N
+    f = ring.frequencies_hz[0] * 0.9713     # an operator key of its own
+    yy, xx = np.meshgrid(np.linspace(-1, 1, ring.ny), np.linspace(-1, 1, ring.nx), indexing="ij")
+    bump = np.exp(-(xx ** 2 + yy ** 2) / 0.1)
+
+    def make(c, mode, factor):
+        return G.HelmholtzSolver(ring.x_m, ring.y_m, c, None, f, -1, ring.cfg["pml_strength"], ring.cfg["pml_m"],
+                                 dict(OPTS, mode=mode, factor=factor))
+
+    m = ring.nx - 2
+    first = make(ring.c_true, "c64", "auto").info()
+    make(ring.c_true, "c64", "auto").close()            # learns the profile, caches the leading pages
+    c2 = ring.c_true * (1 + 2e-3 * bump)
+    seeded = make(c2, "c64", "auto")
+    info = seeded.info()
+    assert info["factor_fallback"] == 0 and info["iterative_blocks"] == m - 1 > first["iterative_blocks"]
+    assert 0 < info["inverse_residual"] <= 1e-4
+    src = _sources(ring, 8)
+    u_s = seeded.solve(src, False)[0]
+    u_f = make(c2, "c64", "fixed").solve(src, False)[0]
+    assert _rel(u_s, u_f) <= 1e-5
+    r_s = make(c2, "refined", "auto").solve(src, True, tol=1e-13)[0]
+    r_f = make(c2, "refined", "fixed").solve(src, True, tol=1e-13)[0]
+    assert _rel(r_s, r_f) <= 1e-12
+    c3 = ring.c_true + 60.0
+    r_s = make(c3, "refined", "auto").solve(src, False, tol=1e-13)[0]
+    r_f = make(c3, "refined", "fixed").solve(src, False, tol=1e-13)[0]
+    assert _rel(r_s, r_f) <= 1e-12
+    N.check(N.load().wust_pool_trim())
+    again = make(ring.c_true, "c64", "auto").info()
+    assert again["iterative_blocks"] < m - 1

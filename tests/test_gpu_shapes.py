@@ -80,7 +80,8 @@ def test_oracle_evaluation_on_ragged_grids(ny, nx):
 
 
 def test_pooled_buffers_reproduce_results():
-    """An operator built on buffers of a destroyed one (the device pool) gives bit-identical results."""
+    """An operator built on buffers of a destroyed one (the device pool) gives bit-identical results
+    (history-free factors: no seeds or learned profile from the earlier operators)."""
     from waveforminversionust_amd import _native as N
     x, y, c = _grid(40, 23)
     f, a0, L = 2.2e5, 10.0, 2e-3
@@ -90,7 +91,7 @@ def test_pooled_buffers_reproduce_results():
     N.check(N.load().wust_pool_trim())
     results = []
     for _ in range(3):
-        s = G.HelmholtzSolver(x, y, c, None, f, -1, a0, L, dict(OPTS, mode="refined", tol=1e-13))
+        s = G.HelmholtzSolver(x, y, c, None, f, -1, a0, L, dict(OPTS, mode="refined", tol=1e-13, factor="fixed"))
         results.append(s.solve(b, False)[0].clone())
         s.close()
         if len(results) == 2:

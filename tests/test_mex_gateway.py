@@ -84,10 +84,15 @@ def test_gateway_matches_python_binding(mex):
     x, y, c = case.x_m, case.y_m, case.c_true
     f = case.frequencies_hz[0]
     ny, nx = c.shape
+    from waveforminversionust_amd import _native as N
     G.HelmholtzSolver(x, y, c, None, f, -1, 10.0, 0.004, OPTS).close()   # both below use the learned factor policy
+    # each create below starts without cached leading pages (the second would otherwise be seeded from
+    # the first factorization, which changes its rounding)
+    N.check(N.load().wust_pool_trim())
     h, info = mex(2, "create", x, y, c, np.zeros((0, 0)), f, -1.0, 10.0, 0.004, np.array([1400.0, 1700.0]), "refined",
                   np.zeros((0, 0)))
     assert h.dtype == np.uint64 and h.shape == (1, 1) and info.shape == (1, 9)
+    N.check(N.load().wust_pool_trim())
     py = G.HelmholtzSolver(x, y, c, None, f, -1, 10.0, 0.004, OPTS)
     assert info[0, 2] == py.stencil_b and info[0, 3] == py.massD and info[0, 4] == py.massE
     rng = np.random.default_rng(9)
@@ -114,10 +119,12 @@ def test_gateway_matches_python_binding(mex):
     init = np.full((ny, nx), 1500.0)
     cfg = dict(case.cfg, tol=1e-12)
     G.wust_oracle(init, obs, 0, cfg, nargout=1)   # the factor policy of this operator, for both below
+    N.check(N.load().wust_pool_trim())
     ho = mex(1, "create", x, y, init, np.zeros((0, 0)), f, -1.0, 10.0, 0.004, np.array([1400.0, 1700.0]), "refined",
              np.zeros((0, 0)))
     loss, g, scales = mex(3, "oracle", ho, case.tx_index.astype(float), case.rx_index.astype(float), Y, mask,
                           2.0, 1.0, 1.0, 1e-12, 30.0)
+    N.check(N.load().wust_pool_trim())
     lw, gw, state = G.wust_oracle(init, obs, 0, cfg, nargout=3)
     assert loss.item() == lw and np.array_equal(g, gw.cpu().numpy())
     assert np.array_equal(scales.ravel(), state.scales)

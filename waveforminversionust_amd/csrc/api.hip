@@ -611,6 +611,7 @@ int wust_pool_trim(void) {
         WUST_HIP_CHECK(hipSetDevice(dev0));
         pl.free_list.clear();
         pl.held = 0;
+        seed_cache_trim();
     });
 }
 

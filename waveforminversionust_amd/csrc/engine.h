@@ -54,6 +54,9 @@ struct PlainLib;
 // ---- factor (twisted block-tridiagonal Schur recurrence + Gauss-Jordan inverses) ----
 // lib: rocSOLVER handle for the pivoted refactorization when a pivot-free block fails its check
 void factor_engine(Operator &op, PlainLib &lib, hipStream_t s);
+// frees the cached leading inverse pages that seed later factorizations with the same key (entries
+// held by a factorization in progress stay)
+void seed_cache_trim();
 // single-chain Schur block T = D_j - L_j X_{j-1} U_{j-1} (T = D_0 for j = 0), X = op.page
 void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s);
 

@@ -674,6 +674,55 @@ struct BudgetKey {
 std::mutex g_budget_mu;
 std::map<BudgetKey, std::vector<uint8_t>> g_budgets;
 
+// Inverse pages of the leading chain steps of the last factorization per (device, operator key).
+// Those steps (the PML columns at the start of each chain) have no usable neighbour seed and go to
+// Gauss-Jordan; their Schur blocks depend only on the model over the columns from the chain's end
+// up to the step, which consecutive evaluations of a reconstruction leave unchanged (the update
+// mask keeps the PML fixed) or change little.  The next factorization with the key seeds
+// Newton-Schulz for those steps with the cached page of the same block instead (two rounds at most;
+// a seed that no longer converges sends that factorization to the Gauss-Jordan fallback and drops
+// the entry).  An entry is held by one factorization at a time; the cache keeps at most
+// WUST_SEED_CACHE_MB megabytes (default 4096), least recently used entries first out.
+struct SeedEntry {
+    cf *buf = nullptr;        // [2 chains][steps][npad * npad]
+    int steps = 0;            // leading chain steps cached per chain
+    size_t bytes = 0;
+    bool busy = false;
+    uint64_t used = 0;        // last use (cache clock)
+};
+std::mutex g_seed_mu;
+std::map<std::pair<int, BudgetKey>, SeedEntry> g_seeds;
+uint64_t g_seed_clock = 0;
+size_t seed_cache_cap() {
+    static const size_t cap = [] {
+        const char *e = std::getenv("WUST_SEED_CACHE_MB");
+        const long long mb = e ? std::atoll(e) : 4096;
+        return (size_t)(mb > 0 ? mb : 0) << 20;
+    }();
+    return cap;
+}
+int current_device_id() {
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return d;
+}
+// releases a held entry on every exit path
+struct SeedHold {
+    SeedEntry *e = nullptr;
+    ~SeedHold() {
+        if (e) {
+            std::lock_guard<std::mutex> lk(g_seed_mu);
+            e->busy = false;
+        }
+    }
+};
+void seed_free(SeedEntry &e) {
+    if (e.buf) (void)hipFree(e.buf);
+    e.buf = nullptr;
+    e.steps = 0;
+    e.bytes = 0;
+}
+
 BudgetKey budget_key(const Operator &op) {
     const AssemblyParams &a = op.ap;
     return BudgetKey{a.ny, a.nx, a.npad, a.dispersion, a.f, a.dx, a.dy, a.a0, a.L};
@@ -688,7 +737,12 @@ struct FactorRun {
     int ns_blocks = 0;
     int update_cap = NS_KMAX;
     std::vector<uint8_t> learned;   // per chain step; empty: the default budget
-
+    const cf *seeds = nullptr;      // cached pages of the leading steps (SeedEntry layout), or null
+    int seed_steps = 0;
+    bool seeded(int st) const { return seeds && st < seed_steps; }
+    const cf *seed_page(int chain, int st) const {
+        return seeds + ((int64_t)chain * seed_steps + st) * op.npad * op.npad;
+    }
     // Newton-Schulz round budget of chain step `st`: the PML columns at the
     // start of each chain change fastest.
     // Newton-Schulz or Gauss-Jordan for chain step `st`.  A Gauss-Jordan block (npad / 32 panel
@@ -719,7 +773,8 @@ struct FactorRun {
         float *ratio_d = reinterpret_cast<float *>(op.status + 2);
         std::vector<int> gj, ns;
         for (int c = 0; c < (int)steps.size(); ++c) {
-            const bool use_ns = allow_ns && op.nswork && steps[c].kind != 2 && st > 0 && ns_step(st);
+            const bool use_ns =
+                allow_ns && op.nswork && steps[c].kind != 2 && (seeded(st) || (st > 0 && ns_step(st)));
             (use_ns ? ns : gj).push_back(c);
         }
         SchurParams sp{};
@@ -804,7 +859,7 @@ struct FactorRun {
             launch_k(PC_NS_GEMM, ns_gemm_kernel<false>, dim3(8 * ((G + 7) / 8)), dim3(256), 0, s, cp);
         }
         if (!ns.empty()) {
-            const int R = ns_budget(st);
+            const int R = seeded(st) ? std::min(2, update_cap) : ns_budget(st);
             const int T = ld / NB;
             const int G = (int)ns.size() * T * T;
             const int grid_ns = 8 * ((G + 7) / 8);
@@ -828,7 +883,9 @@ struct FactorRun {
                     tk.T = w;
                     tk.E = w + pg;
                     tk.M = xb + 2 * pg;
-                    tk.X = (r == 0) ? op.page(steps[c].kind == 0 ? j - 1 : j + 1) : xb + ((r - 1) % 2) * pg;
+                    const cf *seed = seeded(st) ? seed_page(steps[c].kind, st)
+                                                : op.page(steps[c].kind == 0 ? j - 1 : j + 1);
+                    tk.X = (r == 0) ? seed : xb + ((r - 1) % 2) * pg;
                     tk.xnext = xb + (r % 2) * pg;
                     tk.page = op.page(j);
                     tk.norms = op.ns_norm + c * NS_KMAX;
@@ -869,7 +926,73 @@ struct FactorRun {
     }
 };
 
+// After a factorization: keep the pages of the leading steps that the next factorization with this key
+// would give to Gauss-Jordan (under `profile`) or that it seeded (`seeded` steps; -1 after a fallback,
+// which drops the entry), evicting least recently used entries beyond the cache's size.
+void update_seed_cache(const Operator &op, SeedEntry &e, const std::vector<uint8_t> &profile, int seeded,
+                       hipStream_t s) {
+    if (seeded < 0 || profile.empty()) {
+        std::lock_guard<std::mutex> lk(g_seed_mu);
+        seed_free(e);
+        return;
+    }
+    const int most = op.npad >= 800 ? 2 : 3;
+    const int nsteps = std::min(op.p, op.m - 1 - op.p);
+    int lead = 0;
+    while (lead < nsteps && (lead == 0 || lead < seeded || lead >= (int)profile.size() || profile[lead] < 1 ||
+                             profile[lead] > most))
+        ++lead;
+    const int64_t pg = (int64_t)op.npad * op.npad;
+    const size_t bytes = sizeof(cf) * (size_t)(2 * lead) * (size_t)pg;
+    {
+        std::lock_guard<std::mutex> lk(g_seed_mu);
+        if (e.bytes != bytes) seed_free(e);
+        // room for this entry: drop other idle entries, least recently used first
+        size_t total = bytes;
+        for (auto &kv : g_seeds)
+            if (&kv.second != &e) total += kv.second.bytes;
+        while (total > seed_cache_cap()) {
+            SeedEntry *lru = nullptr;
+            for (auto &kv : g_seeds)
+                if (&kv.second != &e && !kv.second.busy && kv.second.bytes && (!lru || kv.second.used < lru->used))
+                    lru = &kv.second;
+            if (!lru) break;
+            total -= lru->bytes;
+            seed_free(*lru);
+        }
+        if (total > seed_cache_cap() || lead == 0) return;
+    }
+    if (!e.buf) {
+        if (hipMalloc(&e.buf, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            e.buf = nullptr;
+            return;   // no room: factor without seeds next time
+        }
+        e.bytes = bytes;
+    }
+    for (int st = 0; st < lead; ++st) {
+        WUST_HIP_CHECK(hipMemcpyAsync(e.buf + (int64_t)st * pg, op.page(st), sizeof(cf) * pg, hipMemcpyDeviceToDevice, s));
+        WUST_HIP_CHECK(hipMemcpyAsync(e.buf + ((int64_t)lead + st) * pg, op.page(op.m - 1 - st), sizeof(cf) * pg,
+                                      hipMemcpyDeviceToDevice, s));
+    }
+    // the entry is released when this returns: the copies land before another factorization reads it
+    WUST_HIP_CHECK(hipStreamSynchronize(s));
+    e.steps = lead;
+}
+
 }  // namespace
+
+void seed_cache_trim() {
+    std::lock_guard<std::mutex> lk(g_seed_mu);
+    for (auto it = g_seeds.begin(); it != g_seeds.end();) {
+        if (it->second.busy) {
+            ++it;
+            continue;
+        }
+        seed_free(it->second);
+        it = g_seeds.erase(it);
+    }
+}
 
 void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
     const uint32_t init[6] = {0u, 0u, 0x3f800000u, 0x3f800000u, 0u, 0u};   // flags, -, ratios 1.0f, residual 0
@@ -892,10 +1015,27 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
         auto it = g_budgets.find(key);
         if (it != g_budgets.end()) run.learned = it->second;
     }
+    // the cached leading pages of this key, when no other factorization holds them
+    const bool seeding = learn && run.allow_ns && op.nswork && op.ns_rounds && seed_cache_cap() > 0;
+    const std::pair<int, BudgetKey> skey{current_device_id(), key};
+    SeedHold hold;
+    if (seeding) {
+        std::lock_guard<std::mutex> lk(g_seed_mu);
+        auto it = g_seeds.find(skey);
+        if (it == g_seeds.end()) it = g_seeds.emplace(skey, SeedEntry{}).first;
+        if (!it->second.busy) {
+            hold.e = &it->second;
+            hold.e->busy = true;
+            hold.e->used = ++g_seed_clock;
+            run.seeds = hold.e->steps > 0 ? hold.e->buf : nullptr;
+            run.seed_steps = run.seeds ? hold.e->steps : 0;
+        }
+    }
     if (op.ns_rounds) WUST_HIP_CHECK(hipMemsetAsync(op.ns_rounds, 0, sizeof(int) * op.m, s));
     run.run();
     uint32_t st[6];
     std::vector<int> used;
+    std::vector<uint8_t> profile;   // the profile left for the next factorization with this key
     if (learn && run.allow_ns && op.ns_rounds) {
         used.resize(op.m);
         WUST_HIP_CHECK(hipMemcpyAsync(used.data(), op.ns_rounds, sizeof(int) * op.m, hipMemcpyDeviceToHost, s));
@@ -909,6 +1049,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
         for (int j = 0; j < m; ++j) {
             if (j == op.p || used[j] == 0) continue;
             const int stp = j < op.p ? j : m - 1 - j;
+            if (run.seeded(stp)) continue;   // rounds from a cached seed say nothing of the neighbour seed
             const bool forced = used[j] >= 64;
             int need = (used[j] & 63) + (forced ? 1 : 0);
             // a step that ran out of rounds needs at least one more than its profile gave it, even
@@ -918,6 +1059,8 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
             need = std::min(need, NS_KMAX);
             next[stp] = (uint8_t)std::max<int>(next[stp], need);
         }
+        for (int stp = 0; stp < run.seed_steps && stp < nsteps; ++stp)
+            next[stp] = stp < (int)run.learned.size() ? run.learned[stp] : 0;
         if (const char *e = std::getenv("WUST_NS_LOG"); e && e[0] == '1') {
             float bound;
             std::memcpy(&bound, st + 4, sizeof(bound));
@@ -926,7 +1069,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
             for (int j = 0; j < m; ++j)
                 if (j != op.p && used[j] >= 64) std::fprintf(stderr, " [block %d forced at %d]", j, used[j] & 63);
             for (int i = 0; i < nsteps && i < 48; ++i) std::fprintf(stderr, " %d", (int)next[i]);
-            std::fprintf(stderr, " ...\n");
+            std::fprintf(stderr, " ... (%d leading steps seeded from the previous factorization)\n", run.seed_steps);
         }
         // An unconverged factorization keeps its profile too: the steps whose blocks ran out of
         // rounds (forced, one more than they had) or needed 4+ reach Gauss-Jordan in the next
@@ -936,6 +1079,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
         // factorization.)
         std::lock_guard<std::mutex> lk(g_budget_mu);
         g_budgets[key] = next;
+        profile = next;
     }
     op.factor_fallback = 0;
     if (st[0] & ST_INVERSE_UNCONVERGED) {
@@ -993,6 +1137,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
         op.min_pivot_ratio = 1.0;
         op.inverse_residual = 0;
     }
+    if (hold.e) update_seed_cache(op, *hold.e, profile, op.factor_fallback == 0 ? run.seed_steps : -1, s);
 }
 
 }  // namespace wust
