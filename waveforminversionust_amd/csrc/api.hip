@@ -309,9 +309,11 @@ struct wust_helmholtz {
     // b -> s32, and the numbers of leading and trailing block columns of b that are zero (point
     // sources and receivers leave the columns between the array and the domain edges empty; the
     // sweeps skip their forward elimination).  px / py (npts nodes, or null): every nonzero of b
-    // lies at one of these nodes, and only they are read.
+    // lies at one of these nodes, and only they are read.  hpx (host copy of px, or null): the
+    // zero columns are those without a node (a node whose values are all zero keeps its column, a
+    // conservative count), found on the host without reading the device's column flags back.
     void demote_rhs(const cd *b, int64_t cnt, int Rpad, hipStream_t s, int &lead, int &trail, const int *px = nullptr,
-                    const int *py = nullptr, int npts = 0) {
+                    const int *py = nullptr, int npts = 0, const int *hpx = nullptr) {
         lead = trail = 0;
         bmask_valid = false;
         const char *e = std::getenv("WUST_SKIP_ZERO_COLUMNS");
@@ -323,6 +325,17 @@ struct wust_helmholtz {
         if (px) sparse_demote_flag(b, s32, cnt, px, py, npts, op.npad, Rpad, op.colflags, bmask, s);
         else demote_flag(b, s32, cnt, (int64_t)op.npad * Rpad, op.colflags, bmask, s);
         bmask_valid = true;
+        if (px && hpx) {
+            int lo = op.m, hi = -1;
+            for (int q = 0; q < npts; ++q)
+                if (hpx[q] >= 0) {
+                    lo = std::min(lo, hpx[q]);
+                    hi = std::max(hi, hpx[q]);
+                }
+            lead = hi < 0 ? op.m : lo;
+            trail = hi < 0 ? op.m : op.m - 1 - hi;
+            return;
+        }
         std::vector<int> f((size_t)op.m);
         WUST_HIP_CHECK(hipMemcpyAsync(f.data(), op.colflags, sizeof(int) * op.m, hipMemcpyDeviceToHost, s));
         WUST_HIP_CHECK(hipStreamSynchronize(s));
@@ -334,7 +347,7 @@ struct wust_helmholtz {
     // nodes, or null): every nonzero of b lies at one of these nodes.
     wust_solve_stats solve_internal(const cd *b, cd *out, int R, int Rpad, bool adjoint, double tol, int max_iter,
                                     bool want_residual, hipStream_t s, const int *px = nullptr,
-                                    const int *py = nullptr, int npts = 0) {
+                                    const int *py = nullptr, int npts = 0, const int *hpx = nullptr) {
         wust_solve_stats st{};
         const int64_t cnt = field_elems(Rpad);
         double *nb2 = norms, *nr2 = norms + Rpad;
@@ -392,7 +405,7 @@ struct wust_helmholtz {
             return st;
         }
         int lead = 0, trail = 0;
-        demote_rhs(b, cnt, Rpad, s, lead, trail, px, py, npts);
+        demote_rhs(b, cnt, Rpad, s, lead, trail, px, py, npts, hpx);
         solve_engine(op, s32, d32, b32, Rpad, adjoint, s, lead, trail);
         if (op.mode == WUST_MODE_C64) {
             promote(d32, out, cnt, s);
@@ -744,7 +757,7 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
             if (retain) ufwd = dalloc<cd>(cnt);
             try {
                 h->solve_internal(h->b64, ufwd, nb, Rpad, false, tol, max_iter, false, s, txx + first, txy + first,
-                                  nb);
+                                  nb, tx_x.data() + first);
                 // source fit, residual, loss and the adjoint source
                 h->clear_status(s);
                 cd *adj = h->b64;  // forward RHS no longer needed: undo its impulses
@@ -777,7 +790,8 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
                     h->scales_host[first + r] = sc[r];
                 }
                 if (need_gradient) {
-                    h->solve_internal(adj, h->lam64, nb, Rpad, true, tol, max_iter, false, s, rxx, rxy, (int)n_rx);
+                    h->solve_internal(adj, h->lam64, nb, Rpad, true, tol, max_iter, false, s, rxx, rxy, (int)n_rx,
+                                      h->rx_x.data());
                     GradientArgs ga{};
                     ga.u = ufwd;
                     ga.lam = h->lam64;

@@ -684,7 +684,7 @@ std::map<BudgetKey, std::vector<uint8_t>> g_budgets;
 // the entry).  An entry is held by one factorization at a time; the cache keeps at most
 // WUST_SEED_CACHE_MB megabytes (default 4096), least recently used entries first out.
 struct SeedEntry {
-    cf *buf = nullptr;        // [2 chains][steps][npad * npad]
+    cf *buf = nullptr;        // [steps pages: blocks 0 .. steps-1][steps pages: blocks m-steps .. m-1]
     int steps = 0;            // leading chain steps cached per chain
     size_t bytes = 0;
     bool busy = false;
@@ -740,8 +740,10 @@ struct FactorRun {
     const cf *seeds = nullptr;      // cached pages of the leading steps (SeedEntry layout), or null
     int seed_steps = 0;
     bool seeded(int st) const { return seeds && st < seed_steps; }
+    // left chain step st is block st, right chain step st is block m-1-st (both halves in block order)
     const cf *seed_page(int chain, int st) const {
-        return seeds + ((int64_t)chain * seed_steps + st) * op.npad * op.npad;
+        const int64_t i = chain == 0 ? st : 2 * (int64_t)seed_steps - 1 - st;
+        return seeds + i * op.npad * op.npad;
     }
     // Newton-Schulz round budget of chain step `st`: the PML columns at the
     // start of each chain change fastest.
@@ -970,11 +972,10 @@ void update_seed_cache(const Operator &op, SeedEntry &e, const std::vector<uint8
         }
         e.bytes = bytes;
     }
-    for (int st = 0; st < lead; ++st) {
-        WUST_HIP_CHECK(hipMemcpyAsync(e.buf + (int64_t)st * pg, op.page(st), sizeof(cf) * pg, hipMemcpyDeviceToDevice, s));
-        WUST_HIP_CHECK(hipMemcpyAsync(e.buf + ((int64_t)lead + st) * pg, op.page(op.m - 1 - st), sizeof(cf) * pg,
-                                      hipMemcpyDeviceToDevice, s));
-    }
+    // the leading pages of each chain are contiguous: blocks 0 .. lead-1 and m-lead .. m-1
+    WUST_HIP_CHECK(hipMemcpyAsync(e.buf, op.page(0), sizeof(cf) * pg * lead, hipMemcpyDeviceToDevice, s));
+    WUST_HIP_CHECK(hipMemcpyAsync(e.buf + (int64_t)lead * pg, op.page(op.m - lead), sizeof(cf) * pg * lead,
+                                  hipMemcpyDeviceToDevice, s));
     // the entry is released when this returns: the copies land before another factorization reads it
     WUST_HIP_CHECK(hipStreamSynchronize(s));
     e.steps = lead;
