@@ -161,8 +161,9 @@ def test_seeded_leading_steps(ring):
     the same key: after a model change that leaves the PML nearly untouched (a smooth bump), the
     seeded factor inverts every block but the meeting one by Newton-Schulz without a fallback, and its
     solutions match a history-free factor of the same model (c64 to FP32 inverse accuracy, refined to
-    1e-12); a large change (the whole model 60 m/s faster) still gives refined solutions to 1e-12
-    (through the fallback where a seed no longer converges); wust_pool_trim drops the seeds."""
+    1e-12); after a large change (the whole model 60 m/s faster) the seeds stop converging, the steps
+    from the first such one are inverted again without seeds (no fallback of the whole factor), and
+    the solutions match the history-free factor the same way; wust_pool_trim drops the seeds."""
     from waveforminversionust_amd import _native as N
     f = ring.frequencies_hz[0] * 0.9713     # an operator key of its own
     yy, xx = np.meshgrid(np.linspace(-1, 1, ring.ny), np.linspace(-1, 1, ring.nx), indexing="ij")
@@ -188,6 +189,10 @@ def test_seeded_leading_steps(ring):
     r_f = make(c2, "refined", "fixed").solve(src, True, tol=1e-13)[0]
     assert _rel(r_s, r_f) <= 1e-12
     c3 = ring.c_true + 60.0
+    stale = make(c3, "c64", "auto")
+    i3 = stale.info()
+    assert i3["factor_fallback"] == 0 and i3["iterative_blocks"] < m - 1
+    assert _rel(stale.solve(src, False)[0], make(c3, "c64", "fixed").solve(src, False)[0]) <= 1e-5
     r_s = make(c3, "refined", "auto").solve(src, False, tol=1e-13)[0]
     r_f = make(c3, "refined", "fixed").solve(src, False, tol=1e-13)[0]
     assert _rel(r_s, r_f) <= 1e-12

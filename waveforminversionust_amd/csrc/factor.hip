@@ -739,6 +739,7 @@ struct FactorRun {
     std::vector<uint8_t> learned;   // per chain step; empty: the default budget
     const cf *seeds = nullptr;      // cached pages of the leading steps (SeedEntry layout), or null
     int seed_steps = 0;
+    int seeded_upto = 0;            // leading steps whose blocks came from the cached seeds
     bool seeded(int st) const { return seeds && st < seed_steps; }
     // left chain step st is block st, right chain step st is block m-1-st (both halves in block order)
     const cf *seed_page(int chain, int st) const {
@@ -913,6 +914,36 @@ struct FactorRun {
         }
     }
 
+    // After the seeded steps: a block whose Newton-Schulz iteration ran out of rounds from its cached
+    // seed (the model changed where the seed came from) hands the chain steps from the first such step
+    // on to the unseeded policy (their Schur blocks depend on it), so that a stale cache costs those
+    // steps, not the whole factor.  Returns the step to resume from (seed_steps when every seeded
+    // block converged).
+    int check_seeds() {
+        const int m = op.m;
+        std::vector<int> used((size_t)m);
+        WUST_HIP_CHECK(hipMemcpyAsync(used.data(), op.ns_rounds, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+        WUST_HIP_CHECK(hipStreamSynchronize(s));
+        int bad = seed_steps;
+        for (int st = 0; st < seed_steps && bad == seed_steps; ++st)
+            if ((used[st] & 64) || (used[m - 1 - st] & 64) || used[st] == 0 || used[m - 1 - st] == 0) bad = st;
+        if (bad == seed_steps) return bad;
+        // only seeded Newton-Schulz steps ran so far: clear the flags and the largest bound that the
+        // failed ones left, and the rounds records of the steps to redo
+        const uint32_t init[6] = {0u, 0u, 0x3f800000u, 0x3f800000u, 0u, 0u};
+        WUST_HIP_CHECK(hipMemcpyAsync(op.status, init, sizeof(init), hipMemcpyHostToDevice, s));
+        const int redo = seed_steps - bad;
+        WUST_HIP_CHECK(hipMemsetAsync(op.ns_rounds + bad, 0, sizeof(int) * redo, s));
+        WUST_HIP_CHECK(hipMemsetAsync(op.ns_rounds + (m - seed_steps), 0, sizeof(int) * redo, s));
+        ns_blocks -= 2 * redo;
+        seeds = nullptr;
+        seeded_upto = bad;
+        if (const char *e = std::getenv("WUST_NS_LOG"); e && e[0] == '1')
+            std::fprintf(stderr, "newton-schulz: the cached seed of chain step %d did not converge; steps %d..%d again "
+                                 "without seeds\n", bad, bad, seed_steps - 1);
+        return bad;
+    }
+
     void run() {
         const int m = op.m, p = op.p;
         // chains: left j = 0..p-1, right j = m-1..p+1, then the middle block p
@@ -923,6 +954,7 @@ struct FactorRun {
             if (st < nl) steps.push_back({st, 0});
             if (st < nr) steps.push_back({m - 1 - st, 1});
             run_steps(steps, st);
+            if (seeds && st == seed_steps - 1) st = check_seeds() - 1;
         }
         run_steps({{p, 2}}, 0);
     }
@@ -1030,6 +1062,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
             hold.e->used = ++g_seed_clock;
             run.seeds = hold.e->steps > 0 ? hold.e->buf : nullptr;
             run.seed_steps = run.seeds ? hold.e->steps : 0;
+            run.seeded_upto = run.seed_steps;
         }
     }
     if (op.ns_rounds) WUST_HIP_CHECK(hipMemsetAsync(op.ns_rounds, 0, sizeof(int) * op.m, s));
@@ -1050,7 +1083,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
         for (int j = 0; j < m; ++j) {
             if (j == op.p || used[j] == 0) continue;
             const int stp = j < op.p ? j : m - 1 - j;
-            if (run.seeded(stp)) continue;   // rounds from a cached seed say nothing of the neighbour seed
+            if (stp < run.seeded_upto) continue;   // rounds from a cached seed say nothing of the neighbour seed
             const bool forced = used[j] >= 64;
             int need = (used[j] & 63) + (forced ? 1 : 0);
             // a step that ran out of rounds needs at least one more than its profile gave it, even
@@ -1060,7 +1093,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
             need = std::min(need, NS_KMAX);
             next[stp] = (uint8_t)std::max<int>(next[stp], need);
         }
-        for (int stp = 0; stp < run.seed_steps && stp < nsteps; ++stp)
+        for (int stp = 0; stp < run.seeded_upto && stp < nsteps; ++stp)
             next[stp] = stp < (int)run.learned.size() ? run.learned[stp] : 0;
         if (const char *e = std::getenv("WUST_NS_LOG"); e && e[0] == '1') {
             float bound;
@@ -1070,7 +1103,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
             for (int j = 0; j < m; ++j)
                 if (j != op.p && used[j] >= 64) std::fprintf(stderr, " [block %d forced at %d]", j, used[j] & 63);
             for (int i = 0; i < nsteps && i < 48; ++i) std::fprintf(stderr, " %d", (int)next[i]);
-            std::fprintf(stderr, " ... (%d leading steps seeded from the previous factorization)\n", run.seed_steps);
+            std::fprintf(stderr, " ... (%d leading steps seeded from the previous factorization)\n", run.seeded_upto);
         }
         // An unconverged factorization keeps its profile too: the steps whose blocks ran out of
         // rounds (forced, one more than they had) or needed 4+ reach Gauss-Jordan in the next
@@ -1138,7 +1171,7 @@ void factor_engine(Operator &op, PlainLib &lib, hipStream_t s) {
         op.min_pivot_ratio = 1.0;
         op.inverse_residual = 0;
     }
-    if (hold.e) update_seed_cache(op, *hold.e, profile, op.factor_fallback == 0 ? run.seed_steps : -1, s);
+    if (hold.e) update_seed_cache(op, *hold.e, profile, op.factor_fallback == 0 ? run.seeded_upto : -1, s);
 }
 
 }  // namespace wust
