@@ -52,12 +52,7 @@ struct CgemmWave {
         cf a[RB][4], b[CB][4];
     };
     v4f p1[RB][CB], p2[RB][CB], p3[RB][CB];   // p2 unused without M3
-#ifdef WUST_FOLD_SEPARATE
-    float q1[FOLD ? RB : 1][FOLD ? CB : 1][4], q2[FOLD && M3 ? RB : 1][FOLD && M3 ? CB : 1][4],
-        q3[FOLD ? RB : 1][FOLD ? CB : 1][4];   // running sums (FOLD)
-#else
     float q1[FOLD ? RB : 1][FOLD ? CB : 1][4], q3[FOLD ? RB : 1][FOLD ? CB : 1][4];   // running sums (FOLD)
-#endif
 
     __device__ __forceinline__ void zero() {
 #pragma unroll
@@ -70,12 +65,7 @@ struct CgemmWave {
 #pragma unroll
                 for (int y = 0; y < CB; ++y)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        q1[x][y][r] = q3[x][y][r] = 0.f;
-#ifdef WUST_FOLD_SEPARATE
-                        if (M3) q2[x][y][r] = 0.f;
-#endif
-                    }
+                    for (int r = 0; r < 4; ++r) q1[x][y][r] = q3[x][y][r] = 0.f;
         }
     }
 
@@ -89,16 +79,10 @@ struct CgemmWave {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         if (M3) {
-#ifdef WUST_FOLD_SEPARATE
-                            q1[x][y][r] += p1[x][y][r];
-                            q2[x][y][r] += p2[x][y][r];
-                            q3[x][y][r] += p3[x][y][r];
-#else
                             // the 3M combination of this k-block, then into the running sums
                             const float a = p1[x][y][r], b = p2[x][y][r], d = p3[x][y][r];
                             q1[x][y][r] += a - b;
                             q3[x][y][r] += d - (a + b);
-#endif
                         } else {
                             q1[x][y][r] += p1[x][y][r];
                             q3[x][y][r] += p3[x][y][r];
@@ -322,13 +306,6 @@ struct CgemmWave {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             if (FOLD) {
-#ifdef WUST_FOLD_SEPARATE
-                if (M3) {
-                    re[r] = q1[rb][cb][r] - q2[rb][cb][r];
-                    im[r] = q3[rb][cb][r] - (q1[rb][cb][r] + q2[rb][cb][r]);
-                    continue;
-                }
-#endif
                 re[r] = q1[rb][cb][r];
                 im[r] = q3[rb][cb][r];
             } else if (M3) {
