@@ -166,7 +166,8 @@ def test_zero_column_skip_edges(monkeypatch, where):
 @pytest.mark.parametrize("adjoint", [False, True])
 def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
     """Every kernel variant of the fused sweep step, forced in turn: they differ only in how the
-    operands reach the MFMA (staged through LDS, buffer or flat loads, prefetch depth) and deal K
+    operands reach the MFMA (staged through LDS, loaded into LDS by the load unit, buffer or flat
+    loads, prefetch depth) and deal K
     over the four waves in the same order, so they give bit-identical solutions, each solving the
     oracle's H to single-precision accuracy.  (52 rows: the last strip is clamped to the block end
     and the padding rows beyond the strips are zero.)"""
@@ -183,7 +184,7 @@ def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
     interior = np.zeros((ny, nx), bool)
     interior[1:-1, 1:-1] = True
     base = None
-    for v in range(7):
+    for v in range(8):
         monkeypatch.setenv("WUST_SWEEP_VARIANT", str(v))
         u = s.solve(b, adjoint)[0].clone()
         if base is None:

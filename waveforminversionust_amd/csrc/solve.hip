@@ -75,11 +75,11 @@ __device__ __forceinline__ cf operand_b(const SweepTask &tk, int k, int r, int n
         const cf *v = st.v + r;
         cf acc;
         if (!st.transposed) {
-            acc = st.cc[k] * v[(int64_t)k * Rpad];
+            acc = cmul_fixed(st.cc[k], v[(int64_t)k * Rpad]);
             if (k >= 1) cfma(acc, st.cn[k], v[(int64_t)(k - 1) * Rpad]);
             if (k + 1 < npad) cfma(acc, st.cs[k], v[(int64_t)(k + 1) * Rpad]);
         } else {
-            acc = conjf_(st.cc[k]) * v[(int64_t)k * Rpad];
+            acc = cmul_fixed(conjf_(st.cc[k]), v[(int64_t)k * Rpad]);
             if (k + 1 < npad) cfma(acc, conjf_(st.cn[k + 1]), v[(int64_t)(k + 1) * Rpad]);
             if (k >= 1) cfma(acc, conjf_(st.cs[k - 1]), v[(int64_t)(k - 1) * Rpad]);
         }
@@ -299,7 +299,7 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
         const cf vs = (k + 1 < npad) ? tile[q][t + 1][cc] : mk(0.f, 0.f);  // row k + 1 (in the tile when < npad)
         // the same products and order as operand_b: P v = c_c v_k + c_n v_{k-1} + c_s v_{k+1}
         // (transposed: conj(cc_k) v_k + conj(cn_{k+1}) v_{k+1} + conj(cs_{k-1}) v_{k-1})
-        cf a = pc * v;
+        cf a = cmul_fixed(pc, v);
         if (!tk.transposed) {
             if (k >= 1) cfma(a, pn, vn);
             if (k + 1 < npad) cfma(a, ps, vs);
@@ -310,6 +310,71 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
         tk.Bn[off] = psrc[q] - a;
     }
 }
+
+// Epilogue operands of a fused step for one thread: the base rows of its wave's tile (pbase, wave 0
+// only), and for its output element (row k, column j0 + cc) the coupling of row k -- pc, then pa
+// and pz in the order of their use (c_n[k], c_s[k]; transposed c_s[k-1], c_n[k+1]), raw -- and the
+// next step's source.  Every lane loads them, with clamped indices, whatever it needs; an absent
+// operand is a zero-extent buffer, which reads as zeros.  Loads under divergent branches (only the
+// owned rows need them) would make the compiler drain every load in flight, the GEMM's first
+// operands included, at the branch joins.  next() forms B' = src' - P v with the products and
+// order of operand_b.
+struct StepEpilogue {
+    typedef float v2f_t __attribute__((ext_vector_type(2)));
+    __amdgpu_buffer_rsrc_t rbase, rcc, rca, rcz, rsrc;
+    uint32_t obase, osrc;
+    int k, ka, kz, npad;
+    bool tr;
+    cf pbase[4], pc, pa, pz, psrc;
+
+    __device__ __forceinline__ static __amdgpu_buffer_rsrc_t rsrc_or_zero(const cf *ptr, int64_t elems, bool use) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(ptr), (short)0, (use && ptr) ? (int)(elems * 8) : 0,
+                                                 0x00020000);
+    }
+    __device__ __forceinline__ static cf ld(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+        const v2f_t u = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+        return mk(u.x, u.y);
+    }
+    __device__ __forceinline__ StepEpilogue(const FusedTask &tk, int npad_, int Rpad, int i0, int j0, int k_, int cc,
+                                            bool wave0) {
+        npad = npad_;
+        k = k_;
+        tr = tk.transposed != 0;
+        const bool next = tk.Bn != nullptr;
+        rbase = rsrc_or_zero(tk.base, (int64_t)npad * Rpad, wave0);
+        rcc = rsrc_or_zero(tk.cc, npad, next);
+        rca = rsrc_or_zero(tr ? tk.cs : tk.cn, npad, next);
+        rcz = rsrc_or_zero(tr ? tk.cn : tk.cs, npad, next);
+        rsrc = rsrc_or_zero(tk.nsrc, (int64_t)npad * Rpad, next);
+        ka = tr ? max(k - 1, 0) : k;
+        kz = tr ? min(k + 1, npad - 1) : k;
+        obase = (uint32_t)(((int64_t)i0 * Rpad + j0) * 8);
+        osrc = (uint32_t)(((int64_t)k * Rpad + j0 + cc) * 8);
+    }
+    // lane (g, c) of the tile loads base rows i0 + 4 g + r of column j0 + c
+    __device__ __forceinline__ void load(int g, int c, int Rpad) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pbase[r] = ld(rbase, obase + (uint32_t)(((4 * g + r) * Rpad + c) * 8));
+        pc = ld(rcc, (uint32_t)k * 8);
+        pa = ld(rca, (uint32_t)ka * 8);
+        pz = ld(rcz, (uint32_t)kz * 8);
+        psrc = ld(rsrc, osrc);
+    }
+    // B' = src' - P v for output element v with row neighbours vn (k - 1) and vs (k + 1)
+    __device__ __forceinline__ cf next(cf v, cf vn, cf vs) const {
+        cf a;
+        if (!tr) {
+            a = cmul_fixed(pc, v);
+            if (k >= 1) cfma(a, pa, vn);
+            if (k + 1 < npad) cfma(a, pz, vs);
+        } else {
+            a = cmul_fixed(conjf_(pc), v);
+            if (k + 1 < npad) cfma(a, conjf_(pz), vs);
+            if (k >= 1) cfma(a, conjf_(pa), vn);
+        }
+        return psrc - a;
+    }
+};
 
 // The 16-row step with operands loaded 16 bytes per lane.  The MFMA wants lane (g, c) to hold row
 // c of A (and column c of B) at k = 4 g + s, which no layout here stores contiguously per lane, so
@@ -398,27 +463,8 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
             }
         }
     };
-    // epilogue operands
-    cf pbase[4];
-    cf pn = mk(0.f, 0.f), pc = mk(0.f, 0.f), ps = mk(0.f, 0.f), psrc = mk(0.f, 0.f);
-    auto load_epilogue = [&] {
-        if (tk.base && ks == 0) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) pbase[r] = tk.base[(int64_t)(i0 + 4 * g + r) * Rpad + j0 + c];
-        }
-        if (own && tk.Bn) {
-            if (!tk.transposed) {
-                pc = tk.cc[k];
-                if (k >= 1) pn = tk.cn[k];
-                if (k + 1 < npad) ps = tk.cs[k];
-            } else {
-                pc = conjf_(tk.cc[k]);
-                if (k >= 1) pn = conjf_(tk.cs[k - 1]);
-                if (k + 1 < npad) ps = conjf_(tk.cn[k + 1]);
-            }
-            psrc = tk.nsrc ? tk.nsrc[(int64_t)k * Rpad + j0 + cc] : mk(0.f, 0.f);
-        }
-    };
+    StepEpilogue ep(tk, npad, Rpad, i0, j0, k, cc, ks == 0);
+    auto load_epilogue = [&] { ep.load(g, c, Rpad); };
 
     using Acc = CgemmWave<1, 1, ALAY, B_ROW, SWEEP_M3, DEPTH, SWEEP_FOLD>;
     Acc acc;
@@ -493,7 +539,7 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             cf v = mk(re[r], im[r]);
-            if (tk.base) v = v + pbase[r];
+            if (tk.base) v = v + ep.pbase[r];
             tile[4 * g + r][c] = v;
         }
     }
@@ -505,16 +551,185 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
     if (!tk.Bn) return;
     const cf vn = (k >= 1) ? tile[t - 1][cc] : mk(0.f, 0.f);
     const cf vs = (k + 1 < npad) ? tile[t + 1][cc] : mk(0.f, 0.f);
-    // the same products and order as operand_b (see sweep_fused_body)
-    cf a = pc * v;
-    if (!tk.transposed) {
-        if (k >= 1) cfma(a, pn, vn);
-        if (k + 1 < npad) cfma(a, ps, vs);
-    } else {
-        if (k + 1 < npad) cfma(a, ps, vs);
-        if (k >= 1) cfma(a, pn, vn);
+    tk.Bn[off] = ep.next(v, vn, vs);
+}
+
+// s_waitcnt immediate that waits until at most n vector memory operations are outstanding
+// (vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt left at their no-wait maxima)
+constexpr unsigned vmcnt_wait(int n) { return (unsigned)((n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8)); }
+
+// The step with A fetched straight into LDS by buffer_load_dwordx4 ... lds: the load unit writes
+// lane l's 16 bytes at the instruction's LDS base + 16 l, so the choice of which 16 bytes of the
+// page each lane fetches lays the k-block out in LDS -- no registers and no LDS write instructions
+// on the way, and deeper prefetch costs LDS only.  A k-block of A (16 x 16 complex, 2 KiB) is 128
+// slots of 16 bytes, filled by two load instructions (slots 0-63, 64-127):
+//   forward (A = X rows i0 .. i0+15, column-major: a slot holds rows 2r, 2r+1 at one k): slot
+//     32 s + 8 g + r holds rows 2r, 2r+1 at k = 4 g + s.  Each 8-lane group of a load reads one
+//     whole 128-byte line (one k), and MFMA step s reads complex 64 s + 16 g + c, 512 contiguous
+//     bytes per wave.
+//   adjoint (A = X^H, so row i of A is column i0 + i of X, k contiguous: a slot holds k-pair p of
+//     one row): slot 8 i + (p ^ (i & 7)) holds k-pair p of row i.  Each load reads 8 whole lines
+//     (rows), and lane (g, c) reads pieces (c, 2g) and (c, 2g + 1): the slots that eight
+//     consecutive lanes read sit in eight different 16-byte bank groups.
+// B is loaded directly into the MFMA lanes (8 bytes per lane; its rows are contiguous along them).
+// The operands are the same values in the same k order as the other variants', so the results
+// are bit-identical to theirs.  Nothing orders these LDS writes against the wave's own later LDS
+// reads, so the loop waits on the vector memory counter itself: a k-block is 2 + 4 loads issued in
+// a fixed order, and when k-block u is read the loads of u + 1 .. u + DEPTH (and, for u = 0, the
+// epilogue operands) are the only ones issued after it.  Each wave's stages are its own (no
+// barrier); after the loop they hold the cross-wave partial tiles and wave 0's the output tile.
+template <int ALAY, int DEPTH>
+__device__ __forceinline__ void sweep_fused_body_dma(const FusedParams &p) {
+    constexpr int KS = 4, NST = DEPTH + 1, VM_PER_KB = 6;
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int L = xcd * p.per_xcd + local;
+    if (L >= p.units) return;
+    const int npad = p.npad, Rpad = p.Rpad;
+    const int Ng = Rpad / 16;
+    const int ng = L % Ng;
+    const int mb = (L / Ng) % p.strips;
+    const int task = L / (Ng * p.strips);
+    const FusedTask &tk = p.task[task];
+    const int o0 = FUSED_OWN * mb;
+    const int o1 = min(o0 + FUSED_OWN - 1, npad - 1);
+    const int i0 = min(max(o0 - 1, 0), npad - 16);
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ks = wave;
+    const int j0 = ng * 16;
+    const int t = (threadIdx.x >> 4) & 15, cc = threadIdx.x & 15;
+    const int k = i0 + t;
+    const bool own = k >= o0 && k <= o1;
+
+    // one LDS array per stage, so that the compiler sees that reading one stage does not depend on
+    // the loads in flight into the others
+    __shared__ float4 stg0[4][128], stg1[4][128], stg2[NST > 2 ? 4 : 1][128], stg3[NST > 3 ? 4 : 1][128];
+    auto stage = [&](int st) -> float4 * {
+        switch (st) {
+            case 0: return stg0[wave];
+            case 1: return stg1[wave];
+            case 2: return stg2[NST > 2 ? wave : 0];
+            default: return stg3[NST > 3 ? wave : 0];
+        }
+    };
+    const int64_t a_off = (ALAY == A_COL) ? i0 : (int64_t)i0 * npad;
+    const auto ra = __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(tk.X + a_off), (short)0,
+                                                      (int)(((int64_t)npad * npad - a_off) * 8), 0x00020000);
+    const auto rbuf = __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(tk.B + j0), (short)0,
+                                                        (int)(((int64_t)npad * Rpad - j0) * 8), 0x00020000);
+    // lane part of the A loads' byte offsets (the instruction and the k-block add wave-uniform parts)
+    const uint32_t va = (ALAY == A_COL)
+        ? (uint32_t)(((4 * ((lane >> 3) & 3) + (lane >> 5)) * npad + 2 * (lane & 7)) * 8)
+        : (uint32_t)(((lane >> 3) * npad + 2 * ((lane & 7) ^ (lane >> 3))) * 8);
+    const uint32_t vbd = (uint32_t)((4 * g * Rpad + c) * 8);   // B[4 g + s][c]
+    typedef float v2f_t __attribute__((ext_vector_type(2)));
+    auto gload = [&](v4f (&b)[2], int st, int k0) {
+        float4 *dst = stage(st);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t sa = (ALAY == A_COL) ? (uint32_t)((k0 + 2 * q) * npad * 8) : (uint32_t)((8 * q * npad + k0) * 8);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (__attribute__((address_space(3))) void *)(dst + 64 * q), 16,
+                                                     va, sa, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const v2f_t u0 = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rbuf, vbd, (uint32_t)((k0 + 2 * q) * Rpad * 8), 0));
+            const v2f_t u1 = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rbuf, vbd, (uint32_t)((k0 + 2 * q + 1) * Rpad * 8), 0));
+            b[q] = v4f{u0.x, u0.y, u1.x, u1.y};
+        }
+    };
+    StepEpilogue ep(tk, npad, Rpad, i0, j0, k, cc, ks == 0);
+    auto load_epilogue = [&] { ep.load(g, c, Rpad); };
+
+    using Acc = CgemmWave<1, 1, ALAY, B_ROW, SWEEP_M3, DEPTH, SWEEP_FOLD>;
+    Acc acc;
+    acc.zero();
+    {
+        const int nkb = npad / 16;
+        const int nk = (nkb - ks + KS - 1) / KS;
+        const int last = ks + (nk - 1) * KS;
+        auto k0 = [&](int u) {
+            const int kb = ks + u * KS;
+            return 16 * (kb < last ? kb : last);
+        };
+        v4f bst[NST][2];
+#pragma unroll
+        for (int u = 0; u < DEPTH; ++u) gload(bst[u], u, k0(u));
+        __builtin_amdgcn_sched_barrier(0);
+        load_epilogue();
+        __builtin_amdgcn_sched_barrier(0);
+        for (int t0 = 0; t0 < nk; t0 += NST) {
+#pragma unroll
+            for (int u = 0; u < NST; ++u) {
+                gload(bst[(u + DEPTH) % NST], (u + DEPTH) % NST, k0(t0 + u + DEPTH));
+                __builtin_amdgcn_sched_barrier(0);
+                if (t0 + u < nk) {
+                    __builtin_amdgcn_s_waitcnt(vmcnt_wait(VM_PER_KB * DEPTH));   // k-block t0 + u is in LDS
+                    __builtin_amdgcn_sched_barrier(0);
+                    typename Acc::Ops o;
+                    if (ALAY == A_COL) {
+                        const cf *la = reinterpret_cast<const cf *>(stage(u));
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) o.a[0][s] = la[64 * s + 16 * g + c];
+                    } else {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const float4 w = stage(u)[8 * c + ((2 * g + h) ^ (c & 7))];
+                            o.a[0][2 * h] = mk(w.x, -w.y);
+                            o.a[0][2 * h + 1] = mk(w.z, -w.w);
+                        }
+                    }
+                    const v4f(&b)[2] = bst[u];
+                    o.b[0][0] = mk(b[0].x, b[0].y);
+                    o.b[0][1] = mk(b[0].z, b[0].w);
+                    o.b[0][2] = mk(b[1].x, b[1].y);
+                    o.b[0][3] = mk(b[1].z, b[1].w);
+                    __builtin_amdgcn_s_setprio(1);   // MFMA phase first (see CgemmWave::run)
+                    acc.mma(o);
+                    __builtin_amdgcn_s_setprio(0);
+                    acc.fold();
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        // the loads past the last k-block (re-reads of it) may still be writing this wave's stages
+        __builtin_amdgcn_s_waitcnt(vmcnt_wait(0));
+        __builtin_amdgcn_sched_barrier(0);
     }
-    tk.Bn[off] = psrc - a;
+
+    // partial tiles of waves 1-3 in their own stage-1 buffers; after the barrier (every wave past its
+    // loop) the output tile in the stage-0 buffers
+    float re[4], im[4];
+    acc.result(0, 0, re, im);
+    if (ks > 0) {
+        stg1[wave][lane] = make_float4(re[0], re[1], re[2], re[3]);
+        stg1[wave][64 + lane] = make_float4(im[0], im[1], im[2], im[3]);
+    }
+    __syncthreads();
+    cf(*tile)[17] = reinterpret_cast<cf(*)[17]>(&stg0[0][0]);
+    if (ks == 0) {
+#pragma unroll
+        for (int w = 1; w < KS; ++w) {
+            const float4 u = stg1[w][lane], v = stg1[w][64 + lane];
+            re[0] += u.x; re[1] += u.y; re[2] += u.z; re[3] += u.w;
+            im[0] += v.x; im[1] += v.y; im[2] += v.z; im[3] += v.w;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            cf v = mk(re[r], im[r]);
+            if (tk.base) v = v + ep.pbase[r];
+            tile[4 * g + r][c] = v;
+        }
+    }
+    __syncthreads();
+    if (!own) return;
+    const int64_t off = (int64_t)k * Rpad + j0 + cc;
+    const cf v = tile[t][cc];
+    tk.out[off] = v;
+    if (!tk.Bn) return;
+    const cf vn = (k >= 1) ? tile[t - 1][cc] : mk(0.f, 0.f);
+    const cf vs = (k + 1 < npad) ? tile[t + 1][cc] : mk(0.f, 0.f);
+    tk.Bn[off] = ep.next(v, vn, vs);
 }
 
 template <int ALAY, int DEPTH, bool LATE, bool BUF>
@@ -524,6 +739,10 @@ __global__ void __launch_bounds__(256) sweep_fused_kernel(FusedParams p) {
 template <int ALAY, int DEPTH, bool LDS_B, bool MID = false>
 __global__ void __launch_bounds__(256) sweep_fused_kernel_lds(FusedParams p) {
     sweep_fused_body_lds<ALAY, DEPTH, LDS_B, MID>(p);
+}
+template <int ALAY, int DEPTH>
+__global__ void __launch_bounds__(256) sweep_fused_kernel_dma(FusedParams p) {
+    sweep_fused_body_dma<ALAY, DEPTH>(p);
 }
 
 // Zeroes rows r0 .. npad-1 of the [npad][Rpad] blocks of two arrays (blockIdx.y selects the
@@ -550,12 +769,15 @@ __global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r
 //   4     flat loads of both, DEPTH 2 (the reference variant)
 //   5, 6  variants 0 and 1 with the epilogue operands loaded right after the first k-block's operands
 //         (their latency hidden by the GEMM, without delaying its first MFMAs)
-constexpr int N_FUSED_VARIANTS = 7;
-constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {1, 1, 2, 1, 2, 1, 1};
+//   7     A loaded straight into LDS (buffer loads to LDS), B direct, epilogue operands early
+//         (deeper prefetch, DEPTH 2 and 3, was never faster at 560^2, 717^2 or 1088^2)
+constexpr int N_FUSED_VARIANTS = 8;
+constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {1, 1, 2, 1, 2, 1, 1, 1};
 const char *const FUSED_KIND[N_FUSED_VARIANTS] = {"A and B staged through LDS", "A staged through LDS, B direct",
                                                   "A and B staged through LDS", "buffer loads", "flat loads",
                                                   "A and B staged through LDS, epilogue operands early",
-                                                  "A staged through LDS, B direct, epilogue operands early"};
+                                                  "A staged through LDS, B direct, epilogue operands early",
+                                                  "A loaded into LDS, B direct, epilogue operands early"};
 
 // strips of 16-row tiles over n rows (each owns FUSED_OWN rows), and the rows they cover
 inline int fused_strips(int n) { return (n + FUSED_OWN - 1) / FUSED_OWN; }
@@ -585,6 +807,7 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
         case 3: go(sweep_fused_kernel<ALAY, 1, true, true>); break;
         case 5: go(sweep_fused_kernel_lds<ALAY, 1, true, true>); break;
         case 6: go(sweep_fused_kernel_lds<ALAY, 1, false, true>); break;
+        case 7: go(sweep_fused_kernel_dma<ALAY, 1>); break;
         default: go(sweep_fused_kernel<ALAY, 2, false, false>); break;
     }
 }

@@ -31,6 +31,12 @@ __host__ __device__ __forceinline__ cf operator*(cf a, cf b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 __host__ __device__ __forceinline__ cf conjf_(cf a) { return {a.re, -a.im}; }
+// a*b with its rounding fixed by the source: each part rounds its first product, then adds the
+// second by one fma (what cfma gives from zero).  Kernels whose results must agree bit for bit use
+// it where operator* would leave the choice of which product to fuse to the compiler.
+__host__ __device__ __forceinline__ cf cmul_fixed(cf a, cf b) {
+    return {fmaf(-a.im, b.im, a.re * b.re), fmaf(a.im, b.re, a.re * b.im)};
+}
 __host__ __device__ __forceinline__ float abs2(cf a) { return a.re * a.re + a.im * a.im; }
 // c += a*b
 __host__ __device__ __forceinline__ void cfma(cf &c, cf a, cf b) {
