@@ -182,7 +182,8 @@ int wust_helmholtz_mass_apply(const wust_helmholtz *h, const void *d_in, void *d
  *   batch:  source batch size (cfg.source_batch_size); <= 0 means all
  *   d_grad: device double [Ny, Nx], written when need_gradient != 0
  *   loss:   host double
- * Source scale, residual and loss are FP64; refinement as in solve(). */
+ * Source scale, residual and loss are FP64; refinement as in solve().  A non-finite loss
+ * or gradient value is WUST_NUMERICAL_FAILURE ("Nonfinite objective/gradient"). */
 int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, const int64_t *rx_index,
                      int64_t n_rx, const void *d_Y, const uint8_t *d_mask, int64_t batch,
                      int need_gradient, int retain, double tol, int max_iter, double *loss,

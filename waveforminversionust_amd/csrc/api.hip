@@ -736,8 +736,12 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
         WUST_HIP_CHECK(hipMemcpyAsync(rxy, h->rx_y.data(), sizeof(int) * n_rx, hipMemcpyHostToDevice, s));
         WUST_HIP_CHECK(hipMemcpyAsync(txx, tx_x.data(), sizeof(int) * n_tx, hipMemcpyHostToDevice, s));
         WUST_HIP_CHECK(hipMemcpyAsync(txy, tx_y.data(), sizeof(int) * n_tx, hipMemcpyHostToDevice, s));
-        if (need_gradient)
+        // status word 6: the gradient's non-finite flag (the factor uses words 0-4)
+        int *nonfinite = reinterpret_cast<int *>(op.status + 6);
+        if (need_gradient) {
             WUST_HIP_CHECK(hipMemsetAsync(d_grad, 0, sizeof(double) * (size_t)op.ny * op.nx, s));
+            WUST_HIP_CHECK(hipMemsetAsync(nonfinite, 0, sizeof(int), s));
+        }
         double total = 0;
         for (int64_t first = 0; first < n_tx; first += batch) {
             const int nb = (int)std::min<int64_t>(batch, n_tx - first);
@@ -828,8 +832,15 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
                 throw;
             }
         }
+        // the reference's closing check (Runtime/matlab/wust_oracle.m:79-80): a finite loss and,
+        // when computed, a finite gradient
+        int grad_bad = 0;
+        if (need_gradient) {
+            launch_nonfinite_flag((const double *)d_grad, (int64_t)op.ny * op.nx, nonfinite, s);
+            WUST_HIP_CHECK(hipMemcpyAsync(&grad_bad, nonfinite, sizeof(int), hipMemcpyDeviceToHost, s));
+        }
         WUST_HIP_CHECK(hipStreamSynchronize(s));
-        if (!std::isfinite(total)) throw Error(WUST_NUMERICAL_FAILURE, "Nonfinite objective");
+        if (!std::isfinite(total) || grad_bad) throw Error(WUST_NUMERICAL_FAILURE, "Nonfinite objective/gradient");
         *loss = total;
     });
 }

@@ -83,6 +83,8 @@ void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int R
 void launch_clear_points(cd *b, const int *px, const int *py, int npts, int R, int npad, int Rpad, hipStream_t s);
 void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s);
 void launch_gradient(const GradientArgs &a, hipStream_t s);
+// *flag |= 1 when any of v[0 .. count) is not finite (the caller zeroes *flag first)
+void launch_nonfinite_flag(const double *v, int64_t count, int *flag, hipStream_t s);
 void launch_curvature_source(const CurvSrcArgs &a, hipStream_t s);
 void launch_curvature_gather(const CurvGatherArgs &a, int R, hipStream_t s);
 void launch_mass_apply(const cd *in, cd *out, int ny, int nx, int64_t R, double d, double e, int adjoint,

@@ -449,6 +449,15 @@ __global__ void __launch_bounds__(256) gradient_kernel(GradientArgs a) {
     }
 }
 
+// flag = 1 when any of v[0 .. count) is not finite (flag zeroed beforehand; only set here)
+__global__ void __launch_bounds__(256) nonfinite_flag_kernel(const double *__restrict__ v, int64_t count,
+                                                             int *__restrict__ flag) {
+    bool bad = false;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        bad |= !isfinite(v[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
 // curvature source: src(i, r) = sum_j K(j - i) vsm(j) u(j, r) delta(j), j interior
 __global__ void __launch_bounds__(256) curvature_source_kernel(CurvSrcArgs a) {
     const int r = blockIdx.x * 16 + (threadIdx.x & 15);
@@ -606,6 +615,12 @@ void launch_gradient(const GradientArgs &a, hipStream_t s) {
     k.cols = (int)std::max<int64_t>(1, std::min<int64_t>(GRAD_COLS, tiles / 2048));
     dim3 grid(a.npad / 8, (a.m + k.cols - 1) / k.cols);
     launch_k(PC_GRADIENT, gradient_kernel, grid, dim3(256), 0, s, k);
+}
+void launch_nonfinite_flag(const double *v, int64_t count, int *flag, hipStream_t s) {
+    if (count <= 0) return;
+    const unsigned blocks = (unsigned)std::min<int64_t>((count + 255) / 256, 1024);
+    hipLaunchKernelGGL(nonfinite_flag_kernel, dim3(blocks), dim3(256), 0, s, v, count, flag);
+    WUST_LAUNCH_CHECK();
 }
 void launch_curvature_source(const CurvSrcArgs &a, hipStream_t s) {
     dim3 grid((a.Rpad + 15) / 16, (a.npad + 15) / 16, a.m);

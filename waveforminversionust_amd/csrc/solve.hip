@@ -314,17 +314,19 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
 // Epilogue operands of a fused step for one thread: the base rows of its wave's tile (pbase, wave 0
 // only), and for its output element (row k, column j0 + cc) the coupling of row k -- pc, then pa
 // and pz in the order of their use (c_n[k], c_s[k]; transposed c_s[k-1], c_n[k+1]), raw -- and the
-// next step's source.  Every lane loads them, with clamped indices, whatever it needs; an absent
-// operand is a zero-extent buffer, which reads as zeros.  Loads under divergent branches (only the
-// owned rows need them) would make the compiler drain every load in flight, the GEMM's first
-// operands included, at the branch joins.  next() forms B' = src' - P v with the products and
-// order of operand_b.
+// next step's source.  Within a wave every lane loads them, with clamped indices, whatever it
+// needs (an absent source is a zero-extent buffer, which reads as zeros): loads under divergent
+// branches (only the owned rows need them) would make the compiler drain every load in flight, the
+// GEMM's first operands included, at the branch joins.  Whether a wave loads them at all depends
+// only on wave-uniform facts (wave 0 of a step with a base; a step with a next operand), so those
+// branches do not diverge.  next() forms B' = src' - P v with the products and order of
+// operand_b.
 struct StepEpilogue {
     typedef float v2f_t __attribute__((ext_vector_type(2)));
     __amdgpu_buffer_rsrc_t rbase, rcc, rca, rcz, rsrc;
     uint32_t obase, osrc;
     int k, ka, kz, npad;
-    bool tr;
+    bool tr, use_base, use_next;
     cf pbase[4], pc, pa, pz, psrc;
 
     __device__ __forceinline__ static __amdgpu_buffer_rsrc_t rsrc_or_zero(const cf *ptr, int64_t elems, bool use) {
@@ -340,12 +342,13 @@ struct StepEpilogue {
         npad = npad_;
         k = k_;
         tr = tk.transposed != 0;
-        const bool next = tk.Bn != nullptr;
-        rbase = rsrc_or_zero(tk.base, (int64_t)npad * Rpad, wave0);
-        rcc = rsrc_or_zero(tk.cc, npad, next);
-        rca = rsrc_or_zero(tr ? tk.cs : tk.cn, npad, next);
-        rcz = rsrc_or_zero(tr ? tk.cn : tk.cs, npad, next);
-        rsrc = rsrc_or_zero(tk.nsrc, (int64_t)npad * Rpad, next);
+        use_base = wave0 && tk.base != nullptr;
+        use_next = tk.Bn != nullptr;
+        rbase = rsrc_or_zero(tk.base, (int64_t)npad * Rpad, use_base);
+        rcc = rsrc_or_zero(tk.cc, npad, use_next);
+        rca = rsrc_or_zero(tr ? tk.cs : tk.cn, npad, use_next);
+        rcz = rsrc_or_zero(tr ? tk.cn : tk.cs, npad, use_next);
+        rsrc = rsrc_or_zero(tk.nsrc, (int64_t)npad * Rpad, use_next);
         ka = tr ? max(k - 1, 0) : k;
         kz = tr ? min(k + 1, npad - 1) : k;
         obase = (uint32_t)(((int64_t)i0 * Rpad + j0) * 8);
@@ -353,12 +356,16 @@ struct StepEpilogue {
     }
     // lane (g, c) of the tile loads base rows i0 + 4 g + r of column j0 + c
     __device__ __forceinline__ void load(int g, int c, int Rpad) {
+        if (use_base) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) pbase[r] = ld(rbase, obase + (uint32_t)(((4 * g + r) * Rpad + c) * 8));
-        pc = ld(rcc, (uint32_t)k * 8);
-        pa = ld(rca, (uint32_t)ka * 8);
-        pz = ld(rcz, (uint32_t)kz * 8);
-        psrc = ld(rsrc, osrc);
+            for (int r = 0; r < 4; ++r) pbase[r] = ld(rbase, obase + (uint32_t)(((4 * g + r) * Rpad + c) * 8));
+        }
+        if (use_next) {
+            pc = ld(rcc, (uint32_t)k * 8);
+            pa = ld(rca, (uint32_t)ka * 8);
+            pz = ld(rcz, (uint32_t)kz * 8);
+            psrc = ld(rsrc, osrc);
+        }
     }
     // B' = src' - P v for output element v with row neighbours vn (k - 1) and vs (k + 1)
     __device__ __forceinline__ cf next(cf v, cf vn, cf vs) const {

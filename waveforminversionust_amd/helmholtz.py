@@ -295,9 +295,7 @@ def wust_oracle(c, obs, fi, cfg, nargout=2):
     elapsed = time.perf_counter() - t0
     if nargout == 1:
         return loss.value
-    g = grad.t()
-    if not bool(torch.all(torch.isfinite(g))):
-        raise WustError("WUST:NumericalFailure", "Nonfinite objective/gradient")
+    g = grad.t()   # the engine has checked loss and gradient for non-finite values
     if nargout == 2:
         return loss.value, g
     scales = np.zeros(tx.size, np.complex128)
