@@ -183,11 +183,16 @@ struct wust_helmholtz {
     // solve layout of a batch of Rpad right-hand sides is the first field_elems(Rpad) elements
     int64_t b64_zero_elems = 0;
     cf *s32 = nullptr, *d32 = nullptr, *d32b = nullptr, *b32 = nullptr;
+    cf *r32 = nullptr;          // refinement residuals, the correction solves' right-hand sides
+    int64_t s32_zero_elems = 0; // leading elements of s32 known to be zero (point right-hand sides
+                                // write their nodes into it and clear them after their solve)
     uint8_t *bmask = nullptr;   // per 64 values of the current right-hand side: any nonzero
     bool bmask_valid = false;
     cf *p_src = nullptr, *p_u = nullptr, *p_tmp = nullptr;
     double *norms = nullptr;    // [2][Rpad]
     double *loss_r = nullptr;   // [Rpad]
+    cd *scales_dev = nullptr;   // source scales of every transmitter of the evaluation: [scales_cap]
+    int64_t scales_cap = 0;
     cd *scale = nullptr;        // [Rpad]
     int *idx = nullptr;         // scratch for node coordinates: 4 * max(ntx, nrx)
     int idx_cap = 0;
@@ -226,6 +231,7 @@ struct wust_helmholtz {
         dfree(op.gj_check);
         dfree(op.status);
         dfree(idx);
+        dfree(scales_dev);
     }
     // the solve workspace of Rpad RHS into the pool (same sizes as ensure_ws allocates)
     void pool_ws() {
@@ -238,6 +244,7 @@ struct wust_helmholtz {
         if (s32) dpool(s32, cnt);
         if (d32) dpool(d32, cnt);
         if (d32b) dpool(d32b, cnt);
+        if (r32) dpool(r32, cnt);
         if (bmask) dpool(bmask, cnt / 64);
         if (b32) dpool(b32, solve_work_elems(op, ws_Rpad));
     }
@@ -256,6 +263,7 @@ struct wust_helmholtz {
         dfree(s32);
         dfree(d32);
         dfree(d32b);
+        dfree(r32);
         dfree(bmask);
         dfree(b32);
         dfree(p_src);
@@ -282,8 +290,12 @@ struct wust_helmholtz {
             p_tmp = dalloc<cf>((int64_t)op.npad * Rpad);
         } else {
             s32 = dalloc<cf>(cnt);
+            s32_zero_elems = 0;
             d32 = dalloc<cf>(cnt);
-            if (op.mode == WUST_MODE_REFINED) d32b = dalloc<cf>(cnt);
+            if (op.mode == WUST_MODE_REFINED) {
+                d32b = dalloc<cf>(cnt);
+                r32 = dalloc<cf>(cnt);
+            }
             bmask = dalloc<uint8_t>(cnt / 64);
             b32 = dalloc<cf>(solve_work_elems(op, Rpad));
         }
@@ -291,6 +303,12 @@ struct wust_helmholtz {
         loss_r = dalloc<double>(Rpad);
         scale = dalloc<cd>(Rpad);
         ws_Rpad = Rpad;
+    }
+    void ensure_scales(int64_t count) {
+        if (count <= scales_cap) return;
+        dfree(scales_dev);
+        scales_dev = dalloc<cd>(count);
+        scales_cap = count;
     }
     void ensure_idx(int count) {
         if (count <= idx_cap) return;
@@ -304,7 +322,6 @@ struct wust_helmholtz {
         WUST_HIP_CHECK(hipStreamSynchronize(s));
         return st;
     }
-    void clear_status(hipStream_t s) { WUST_HIP_CHECK(hipMemsetAsync(op.status, 0, sizeof(uint32_t), s)); }
 
     // b -> s32, and the numbers of leading and trailing block columns of b that are zero (point
     // sources and receivers leave the columns between the array and the domain edges empty; the
@@ -319,11 +336,19 @@ struct wust_helmholtz {
         const char *e = std::getenv("WUST_SKIP_ZERO_COLUMNS");
         if (!op.colflags || !bmask || (e && e[0] == '0')) {
             demote(b, s32, cnt, s);
+            s32_zero_elems = 0;
             return;
         }
         WUST_HIP_CHECK(hipMemsetAsync(op.colflags, 0, sizeof(int) * op.m, s));
-        if (px) sparse_demote_flag(b, s32, cnt, px, py, npts, op.npad, Rpad, op.colflags, bmask, s);
-        else demote_flag(b, s32, cnt, (int64_t)op.npad * Rpad, op.colflags, bmask, s);
+        if (px) {
+            // s32 stays zero between point right-hand sides (their nodes are cleared after the solve),
+            // so only its first use, or a dense use before, needs the full clear
+            sparse_demote_flag(b, s32, cnt, px, py, npts, op.npad, Rpad, op.colflags, bmask, s, s32_zero_elems >= cnt);
+            s32_zero_elems = cnt;
+        } else {
+            demote_flag(b, s32, cnt, (int64_t)op.npad * Rpad, op.colflags, bmask, s);
+            s32_zero_elems = 0;
+        }
         bmask_valid = true;
         if (px && hpx) {
             int lo = op.m, hi = -1;
@@ -407,6 +432,7 @@ struct wust_helmholtz {
         int lead = 0, trail = 0;
         demote_rhs(b, cnt, Rpad, s, lead, trail, px, py, npts, hpx);
         solve_engine(op, s32, d32, b32, Rpad, adjoint, s, lead, trail);
+        if (px && s32_zero_elems >= cnt) launch_clear_points32(s32, px, py, npts, op.npad, Rpad, s);
         if (op.mode == WUST_MODE_C64) {
             promote(d32, out, cnt, s);
             if (want_residual) st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr, nullptr);
@@ -418,21 +444,21 @@ struct wust_helmholtz {
         // later passes accumulate in FP64, alternating between two buffers ordered so that the
         // usual single correction ends in `out` without a copy.
         cd *bufs[2] = {t64 == out ? u64 : t64, out};
-        double rel = residual_pass(nullptr, nullptr, d32, nullptr, s32);
+        double rel = residual_pass(nullptr, nullptr, d32, nullptr, r32);
         st.initial_residual = st.final_residual = rel;
         int it = 0, cur = 1;
         if (!(rel > tol) || max_iter <= 0) {
             promote(d32, out, cnt, s);
         } else {
-            solve_engine(op, s32, d32b, b32, Rpad, adjoint, s);
+            solve_engine(op, r32, d32b, b32, Rpad, adjoint, s);
             it = 1;
-            rel = st.final_residual = residual_pass(nullptr, d32, d32b, bufs[cur], s32);
+            rel = st.final_residual = residual_pass(nullptr, d32, d32b, bufs[cur], r32);
             while (rel > tol && it < max_iter) {
-                solve_engine(op, s32, d32, b32, Rpad, adjoint, s);
+                solve_engine(op, r32, d32, b32, Rpad, adjoint, s);
                 ++it;
                 const cd *u_old = bufs[cur];
                 cur ^= 1;
-                rel = st.final_residual = residual_pass(u_old, nullptr, d32, bufs[cur], s32);
+                rel = st.final_residual = residual_pass(u_old, nullptr, d32, bufs[cur], r32);
             }
             if (bufs[cur] != out)
                 WUST_HIP_CHECK(hipMemcpyAsync(out, bufs[cur], sizeof(cd) * cnt, hipMemcpyDeviceToDevice, s));
@@ -563,7 +589,7 @@ int wust_helmholtz_create(const wust_grid_desc *desc, const double *d_vel, const
         op.vsm = dalloc<cd>(nodes);
         op.w64 = dalloc<cd>(9 * (int64_t)op.m * op.n);
         op.w32 = dalloc<cf>(9 * (int64_t)op.m * op.npad);
-        op.status = dalloc<uint32_t>(8);
+        op.status = dalloc<uint32_t>(16);
         const int64_t pg = (int64_t)op.npad * op.npad;
         op.pages = dalloc<cf>(pg * op.m);
         op.work = dalloc<cf>(6 * pg);
@@ -736,13 +762,15 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
         WUST_HIP_CHECK(hipMemcpyAsync(rxy, h->rx_y.data(), sizeof(int) * n_rx, hipMemcpyHostToDevice, s));
         WUST_HIP_CHECK(hipMemcpyAsync(txx, tx_x.data(), sizeof(int) * n_tx, hipMemcpyHostToDevice, s));
         WUST_HIP_CHECK(hipMemcpyAsync(txy, tx_y.data(), sizeof(int) * n_tx, hipMemcpyHostToDevice, s));
-        // status word 6: the gradient's non-finite flag (the factor uses words 0-4)
+        // status words 6-9 (the factor uses 0-4): the gradient's non-finite flag, the source fits'
+        // failure flags and the loss summed over the batches (a double), read back once at the end
         int *nonfinite = reinterpret_cast<int *>(op.status + 6);
-        if (need_gradient) {
+        uint32_t *fit_status = op.status + 7;
+        double *loss_acc = reinterpret_cast<double *>(op.status + 8);
+        WUST_HIP_CHECK(hipMemsetAsync(op.status + 6, 0, 4 * sizeof(uint32_t), s));
+        if (need_gradient)
             WUST_HIP_CHECK(hipMemsetAsync(d_grad, 0, sizeof(double) * (size_t)op.ny * op.nx, s));
-            WUST_HIP_CHECK(hipMemsetAsync(nonfinite, 0, sizeof(int), s));
-        }
-        double total = 0;
+        h->ensure_scales(n_tx);
         for (int64_t first = 0; first < n_tx; first += batch) {
             const int nb = (int)std::min<int64_t>(batch, n_tx - first);
             const int Rpad = (int)round_up(nb, 16);
@@ -763,7 +791,6 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
                 h->solve_internal(h->b64, ufwd, nb, Rpad, false, tol, max_iter, false, s, txx + first, txy + first,
                                   nb, tx_x.data() + first);
                 // source fit, residual, loss and the adjoint source
-                h->clear_status(s);
                 cd *adj = h->b64;  // forward RHS no longer needed: undo its impulses
                 launch_impulses(adj, txx + first, txy + first, nb, op.npad, Rpad, s, 0.0);
                 SourceFitArgs fa{};
@@ -775,24 +802,19 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
                 fa.scale = h->scale;
                 fa.loss = h->loss_r;
                 fa.adj_src = need_gradient ? adj : nullptr;
-                fa.status = op.status;
+                fa.status = fit_status;
                 fa.ntx = (int)n_tx;
                 fa.nrx = (int)n_rx;
                 fa.tx_offset = (int)first;
                 fa.npad = op.npad;
                 fa.Rpad = Rpad;
                 launch_source_fit(fa, nb, s);
-                std::vector<double> lr(nb);
-                std::vector<cd> sc(nb);
-                WUST_HIP_CHECK(hipMemcpyAsync(lr.data(), h->loss_r, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-                WUST_HIP_CHECK(hipMemcpyAsync(sc.data(), h->scale, sizeof(cd) * nb, hipMemcpyDeviceToHost, s));
-                uint32_t st = h->read_status(s);
-                if (st & ST_SCALE_FAILURE)
-                    throw Error(WUST_NUMERICAL_FAILURE, "Unusable source-scale denominator/numerator");
-                for (int r = 0; r < nb; ++r) {
-                    total += lr[r];
-                    h->scales_host[first + r] = sc[r];
-                }
+                // the batch's losses into the total in RHS order, its scales beside the others'; a
+                // failed fit is reported at the end (Runtime/matlab/wust_oracle.m:59 checks each
+                // batch: the result is the same error, after the remaining batches' work)
+                launch_accumulate(h->loss_r, nb, loss_acc, s);
+                WUST_HIP_CHECK(hipMemcpyAsync(h->scales_dev + first, h->scale, sizeof(cd) * nb,
+                                              hipMemcpyDeviceToDevice, s));
                 if (need_gradient) {
                     h->solve_internal(adj, h->lam64, nb, Rpad, true, tol, max_iter, false, s, rxx, rxy, (int)n_rx,
                                       h->rx_x.data());
@@ -834,13 +856,17 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
         }
         // the reference's closing check (Runtime/matlab/wust_oracle.m:79-80): a finite loss and,
         // when computed, a finite gradient
-        int grad_bad = 0;
-        if (need_gradient) {
-            launch_nonfinite_flag((const double *)d_grad, (int64_t)op.ny * op.nx, nonfinite, s);
-            WUST_HIP_CHECK(hipMemcpyAsync(&grad_bad, nonfinite, sizeof(int), hipMemcpyDeviceToHost, s));
-        }
+        if (need_gradient) launch_nonfinite_flag((const double *)d_grad, (int64_t)op.ny * op.nx, nonfinite, s);
+        uint32_t words[4];   // status words 6-9
+        WUST_HIP_CHECK(hipMemcpyAsync(words, op.status + 6, sizeof(words), hipMemcpyDeviceToHost, s));
+        WUST_HIP_CHECK(hipMemcpyAsync(h->scales_host.data(), h->scales_dev, sizeof(cd) * n_tx, hipMemcpyDeviceToHost,
+                                      s));
         WUST_HIP_CHECK(hipStreamSynchronize(s));
-        if (!std::isfinite(total) || grad_bad) throw Error(WUST_NUMERICAL_FAILURE, "Nonfinite objective/gradient");
+        if (words[1] & ST_SCALE_FAILURE)
+            throw Error(WUST_NUMERICAL_FAILURE, "Unusable source-scale denominator/numerator");
+        double total;
+        std::memcpy(&total, words + 2, sizeof(total));
+        if (!std::isfinite(total) || words[0] != 0) throw Error(WUST_NUMERICAL_FAILURE, "Nonfinite objective/gradient");
         *loss = total;
     });
 }

@@ -36,8 +36,9 @@ struct Operator {
     int *colflags = nullptr;      // per block column: 1 when a right-hand side has a nonzero there: [m]
     float *gj_check = nullptr;    // ||I - T_j X_j||_F^2 of each Gauss-Jordan block (0 elsewhere): [m]
     uint32_t *status = nullptr;   // device status words: [0] flags, [2..3] pivot ratios (float),
-                                  // [4] largest Newton-Schulz residual bound (float), [6] a gradient
-                                  // value is not finite (wust_oracle_eval)
+                                  // [4] largest Newton-Schulz residual bound (float); wust_oracle_eval:
+                                  // [6] a gradient value is not finite, [7] source-fit failure flags,
+                                  // [8..9] the loss summed over batches (double)
     double assembly_ms = 0, factor_ms = 0, min_pivot_ratio = 1.0;
     double inverse_residual = 0;  // max over Newton-Schulz blocks of the bound on ||I - T_j X_j||_F
     int ns_blocks = 0;            // blocks inverted by Newton-Schulz

@@ -73,9 +73,14 @@ void demote(const cd *in, cf *out, int64_t count, hipStream_t s);
 // a nonzero (count a multiple of 64)
 void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, uint8_t *mask, hipStream_t s);
 // the same for a right-hand side whose nonzeros all lie at the nodes (px[q], py[q]) (px < 0: none),
-// reading only those nodes (flags pre-zeroed; count a multiple of 64)
+// reading only those nodes (flags pre-zeroed; count a multiple of 64); out is zeroed first unless
+// out_zero says that it already is
 void sparse_demote_flag(const cd *in, cf *out, int64_t count, const int *px, const int *py, int npts, int npad,
-                        int Rpad, int *flags, uint8_t *mask, hipStream_t s);
+                        int Rpad, int *flags, uint8_t *mask, hipStream_t s, bool out_zero = false);
+// b = 0 at nodes (px[q], py[q]) for every RHS r < Rpad, single precision (px[q] < 0: no node)
+void launch_clear_points32(cf *b, const int *px, const int *py, int npts, int npad, int Rpad, hipStream_t s);
+// *acc += v[0] + ... + v[n-1], added one at a time in index order on the device
+void launch_accumulate(const double *v, int n, double *acc, hipStream_t s);
 // b at node (sx[r], sy[r]) of RHS r = value, r < R (value 0 undoes the impulses)
 void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int Rpad, hipStream_t s,
                      double value = 1.0);

@@ -304,6 +304,23 @@ __global__ void clear_points_kernel(cd *__restrict__ b, const int *__restrict__ 
     b[((int64_t)px[q] * npad + py[q]) * Rpad + r] = mkd(0, 0);
 }
 
+// zeroes b at nodes (px[q], py[q]) for every RHS r < Rpad (single precision; px[q] < 0: no node)
+__global__ void clear_points32_kernel(cf *__restrict__ b, const int *__restrict__ px, const int *__restrict__ py,
+                                      int npts, int npad, int Rpad) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+    if (r >= Rpad || q >= npts || px[q] < 0) return;
+    b[((int64_t)px[q] * npad + py[q]) * Rpad + r] = mk(0.f, 0.f);
+}
+
+// *acc += v[0] + v[1] + ... + v[n-1], one addition at a time in index order (one thread: the
+// same sum, rounding for rounding, as a host loop over the values)
+__global__ void accumulate_kernel(const double *__restrict__ v, int n, double *__restrict__ acc) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double t = *acc;
+    for (int i = 0; i < n; ++i) t += v[i];
+    *acc = t;
+}
+
 // ---------------------------------------------------------------------------
 // source fit (Runtime/matlab/wust_oracle.m:43-64): one block per RHS
 // ---------------------------------------------------------------------------
@@ -589,9 +606,9 @@ void launch_impulses(cd *b, const int *sx, const int *sy, int R, int npad, int R
     WUST_LAUNCH_CHECK();
 }
 void sparse_demote_flag(const cd *in, cf *out, int64_t count, const int *px, const int *py, int npts, int npad,
-                        int Rpad, int *flags, uint8_t *mask, hipStream_t s) {
+                        int Rpad, int *flags, uint8_t *mask, hipStream_t s, bool out_zero) {
     require(count % 64 == 0, "sparse_demote_flag: size must be a multiple of 64");
-    WUST_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(cf) * count, s));
+    if (!out_zero) WUST_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(cf) * count, s));
     WUST_HIP_CHECK(hipMemsetAsync(mask, 0, count / 64, s));
     if (npts <= 0) return;
     hipLaunchKernelGGL(sparse_demote_flag_kernel, dim3((Rpad + 255) / 256, npts), dim3(256), 0, s, in, out, px, py,
@@ -602,6 +619,16 @@ void launch_clear_points(cd *b, const int *px, const int *py, int npts, int R, i
     if (npts <= 0 || R <= 0) return;
     hipLaunchKernelGGL(clear_points_kernel, dim3((R + 255) / 256, npts), dim3(256), 0, s, b, px, py, npts, R, npad,
                        Rpad);
+    WUST_LAUNCH_CHECK();
+}
+void launch_clear_points32(cf *b, const int *px, const int *py, int npts, int npad, int Rpad, hipStream_t s) {
+    if (npts <= 0 || Rpad <= 0) return;
+    hipLaunchKernelGGL(clear_points32_kernel, dim3((Rpad + 255) / 256, npts), dim3(256), 0, s, b, px, py, npts, npad,
+                       Rpad);
+    WUST_LAUNCH_CHECK();
+}
+void launch_accumulate(const double *v, int n, double *acc, hipStream_t s) {
+    hipLaunchKernelGGL(accumulate_kernel, dim3(1), dim3(64), 0, s, v, n, acc);
     WUST_LAUNCH_CHECK();
 }
 void launch_source_fit(const SourceFitArgs &a, int R, hipStream_t s) {
