@@ -184,7 +184,7 @@ def test_sweep_step_variants_agree(monkeypatch, ny, nx, adjoint):
     interior = np.zeros((ny, nx), bool)
     interior[1:-1, 1:-1] = True
     base = None
-    for v in range(8):
+    for v in range(9):
         monkeypatch.setenv("WUST_SWEEP_VARIANT", str(v))
         u = s.solve(b, adjoint)[0].clone()
         if base is None:

@@ -578,16 +578,19 @@ constexpr unsigned vmcnt_wait(int n) { return (unsigned)((n & 15) | ((n >> 4) <<
 //     one row): slot 8 i + (p ^ (i & 7)) holds k-pair p of row i.  Each load reads 8 whole lines
 //     (rows), and lane (g, c) reads pieces (c, 2g) and (c, 2g + 1): the slots that eight
 //     consecutive lanes read sit in eight different 16-byte bank groups.
-// B is loaded directly into the MFMA lanes (8 bytes per lane; its rows are contiguous along them).
-// The operands are the same values in the same k order as the other variants', so the results
-// are bit-identical to theirs.  Nothing orders these LDS writes against the wave's own later LDS
-// reads, so the loop waits on the vector memory counter itself: a k-block is 2 + 4 loads issued in
-// a fixed order, and when k-block u is read the loads of u + 1 .. u + DEPTH (and, for u = 0, the
-// epilogue operands) are the only ones issued after it.  Each wave's stages are its own (no
+// B is loaded directly into the MFMA lanes (8 bytes per lane; its rows are contiguous along them),
+// or with DMA_B into a stage of its own the same way as the forward A (a k-line of B holds 16
+// columns: slot 32 s + 8 g + r holds columns 2r, 2r+1 at k = 4 g + s), so that a k-block
+// takes four load instructions, all of whole lines.  The operands are the same values in the same
+// k order as the other variants', so the results are bit-identical to theirs.  Nothing orders
+// these LDS writes against the wave's own later LDS reads, so the loop waits on the vector memory
+// counter itself: a k-block is 2 + 4 (DMA_B: 2 + 2) loads issued in a fixed order, and when k-block
+// u is read the loads of u + 1 .. u + DEPTH (and, for u = 0, the epilogue operands) are the only
+// ones issued after it.  Each wave's stages are its own (no
 // barrier); after the loop they hold the cross-wave partial tiles and wave 0's the output tile.
-template <int ALAY, int DEPTH>
+template <int ALAY, int DEPTH, bool DMA_B = false>
 __device__ __forceinline__ void sweep_fused_body_dma(const FusedParams &p) {
-    constexpr int KS = 4, NST = DEPTH + 1, VM_PER_KB = 6;
+    constexpr int KS = 4, NST = DEPTH + 1, VM_PER_KB = DMA_B ? 4 : 6;
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
     const int L = xcd * p.per_xcd + local;
     if (L >= p.units) return;
@@ -608,15 +611,26 @@ __device__ __forceinline__ void sweep_fused_body_dma(const FusedParams &p) {
     const int k = i0 + t;
     const bool own = k >= o0 && k <= o1;
 
-    // one LDS array per stage, so that the compiler sees that reading one stage does not depend on
-    // the loads in flight into the others
+    // one LDS array per stage and operand, so that the compiler sees that reading one does not
+    // depend on the loads in flight into the others
     __shared__ float4 stg0[4][128], stg1[4][128], stg2[NST > 2 ? 4 : 1][128], stg3[NST > 3 ? 4 : 1][128];
+    __shared__ float4 stb0[DMA_B ? 4 : 1][128], stb1[DMA_B ? 4 : 1][128], stb2[DMA_B && NST > 2 ? 4 : 1][128],
+        stb3[DMA_B && NST > 3 ? 4 : 1][128];
     auto stage = [&](int st) -> float4 * {
         switch (st) {
             case 0: return stg0[wave];
             case 1: return stg1[wave];
             case 2: return stg2[NST > 2 ? wave : 0];
             default: return stg3[NST > 3 ? wave : 0];
+        }
+    };
+    auto bstage = [&](int st) -> float4 * {
+        const int w = DMA_B ? wave : 0;
+        switch (st) {
+            case 0: return stb0[w];
+            case 1: return stb1[w];
+            case 2: return stb2[DMA_B && NST > 2 ? w : 0];
+            default: return stb3[DMA_B && NST > 3 ? w : 0];
         }
     };
     const int64_t a_off = (ALAY == A_COL) ? i0 : (int64_t)i0 * npad;
@@ -629,6 +643,8 @@ __device__ __forceinline__ void sweep_fused_body_dma(const FusedParams &p) {
         ? (uint32_t)(((4 * ((lane >> 3) & 3) + (lane >> 5)) * npad + 2 * (lane & 7)) * 8)
         : (uint32_t)(((lane >> 3) * npad + 2 * ((lane & 7) ^ (lane >> 3))) * 8);
     const uint32_t vbd = (uint32_t)((4 * g * Rpad + c) * 8);   // B[4 g + s][c]
+    // DMA_B: B's k-block in slots 128-255 in the forward A's layout (a k-line of B is 16 columns)
+    const uint32_t vbl = (uint32_t)(((4 * ((lane >> 3) & 3) + (lane >> 5)) * Rpad + 2 * (lane & 7)) * 8);
     typedef float v2f_t __attribute__((ext_vector_type(2)));
     auto gload = [&](v4f (&b)[2], int st, int k0) {
         float4 *dst = stage(st);
@@ -638,11 +654,18 @@ __device__ __forceinline__ void sweep_fused_body_dma(const FusedParams &p) {
             __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (__attribute__((address_space(3))) void *)(dst + 64 * q), 16,
                                                      va, sa, 0, 0);
         }
+        if (DMA_B) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const v2f_t u0 = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rbuf, vbd, (uint32_t)((k0 + 2 * q) * Rpad * 8), 0));
-            const v2f_t u1 = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rbuf, vbd, (uint32_t)((k0 + 2 * q + 1) * Rpad * 8), 0));
-            b[q] = v4f{u0.x, u0.y, u1.x, u1.y};
+            for (int q = 0; q < 2; ++q)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rbuf, (__attribute__((address_space(3))) void *)(bstage(st) + 64 * q),
+                                                         16, vbl, (uint32_t)((k0 + 2 * q) * Rpad * 8), 0, 0);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const v2f_t u0 = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rbuf, vbd, (uint32_t)((k0 + 2 * q) * Rpad * 8), 0));
+                const v2f_t u1 = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(rbuf, vbd, (uint32_t)((k0 + 2 * q + 1) * Rpad * 8), 0));
+                b[q] = v4f{u0.x, u0.y, u1.x, u1.y};
+            }
         }
     };
     StepEpilogue ep(tk, npad, Rpad, i0, j0, k, cc, ks == 0);
@@ -686,11 +709,17 @@ __device__ __forceinline__ void sweep_fused_body_dma(const FusedParams &p) {
                             o.a[0][2 * h + 1] = mk(w.z, -w.w);
                         }
                     }
-                    const v4f(&b)[2] = bst[u];
-                    o.b[0][0] = mk(b[0].x, b[0].y);
-                    o.b[0][1] = mk(b[0].z, b[0].w);
-                    o.b[0][2] = mk(b[1].x, b[1].y);
-                    o.b[0][3] = mk(b[1].z, b[1].w);
+                    if (DMA_B) {
+                        const cf *lb = reinterpret_cast<const cf *>(bstage(u));
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) o.b[0][s] = lb[64 * s + 16 * g + c];
+                    } else {
+                        const v4f(&b)[2] = bst[u];
+                        o.b[0][0] = mk(b[0].x, b[0].y);
+                        o.b[0][1] = mk(b[0].z, b[0].w);
+                        o.b[0][2] = mk(b[1].x, b[1].y);
+                        o.b[0][3] = mk(b[1].z, b[1].w);
+                    }
                     __builtin_amdgcn_s_setprio(1);   // MFMA phase first (see CgemmWave::run)
                     acc.mma(o);
                     __builtin_amdgcn_s_setprio(0);
@@ -747,9 +776,9 @@ template <int ALAY, int DEPTH, bool LDS_B, bool MID = false>
 __global__ void __launch_bounds__(256) sweep_fused_kernel_lds(FusedParams p) {
     sweep_fused_body_lds<ALAY, DEPTH, LDS_B, MID>(p);
 }
-template <int ALAY, int DEPTH>
+template <int ALAY, int DEPTH, bool DMA_B>
 __global__ void __launch_bounds__(256) sweep_fused_kernel_dma(FusedParams p) {
-    sweep_fused_body_dma<ALAY, DEPTH>(p);
+    sweep_fused_body_dma<ALAY, DEPTH, DMA_B>(p);
 }
 
 // Zeroes rows r0 .. npad-1 of the [npad][Rpad] blocks of two arrays (blockIdx.y selects the
@@ -778,13 +807,15 @@ __global__ void pad_rows_zero_kernel(cf *a, int64_t na, cf *b, int64_t nb, int r
 //         (their latency hidden by the GEMM, without delaying its first MFMAs)
 //   7     A loaded straight into LDS (buffer loads to LDS), B direct, epilogue operands early
 //         (deeper prefetch, DEPTH 2 and 3, was never faster at 560^2, 717^2 or 1088^2)
-constexpr int N_FUSED_VARIANTS = 8;
-constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {1, 1, 2, 1, 2, 1, 1, 1};
+//   8     A and B loaded straight into LDS, epilogue operands early
+constexpr int N_FUSED_VARIANTS = 9;
+constexpr int FUSED_DEPTH[N_FUSED_VARIANTS] = {1, 1, 2, 1, 2, 1, 1, 1, 1};
 const char *const FUSED_KIND[N_FUSED_VARIANTS] = {"A and B staged through LDS", "A staged through LDS, B direct",
                                                   "A and B staged through LDS", "buffer loads", "flat loads",
                                                   "A and B staged through LDS, epilogue operands early",
                                                   "A staged through LDS, B direct, epilogue operands early",
-                                                  "A loaded into LDS, B direct, epilogue operands early"};
+                                                  "A loaded into LDS, B direct, epilogue operands early",
+                                                  "A and B loaded into LDS, epilogue operands early"};
 
 // strips of 16-row tiles over n rows (each owns FUSED_OWN rows), and the rows they cover
 inline int fused_strips(int n) { return (n + FUSED_OWN - 1) / FUSED_OWN; }
@@ -814,7 +845,8 @@ void launch_fused_variant(int v, const FusedParams &fp, hipStream_t s, bool prof
         case 3: go(sweep_fused_kernel<ALAY, 1, true, true>); break;
         case 5: go(sweep_fused_kernel_lds<ALAY, 1, true, true>); break;
         case 6: go(sweep_fused_kernel_lds<ALAY, 1, false, true>); break;
-        case 7: go(sweep_fused_kernel_dma<ALAY, 1>); break;
+        case 7: go(sweep_fused_kernel_dma<ALAY, 1, false>); break;
+        case 8: go(sweep_fused_kernel_dma<ALAY, 1, true>); break;
         default: go(sweep_fused_kernel<ALAY, 2, false, false>); break;
     }
 }
