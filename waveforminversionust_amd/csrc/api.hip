@@ -186,6 +186,8 @@ struct wust_helmholtz {
     cf *r32 = nullptr;          // refinement residuals, the correction solves' right-hand sides
     int64_t s32_zero_elems = 0; // leading elements of s32 known to be zero (point right-hand sides
                                 // write their nodes into it and clear them after their solve)
+    int64_t s32_points_elems = 0; // nonzero while s32 holds a point right-hand side's nodes: the
+                                  // zero extent to restore once they are cleared
     uint8_t *bmask = nullptr;   // per 64 values of the current right-hand side: any nonzero
     bool bmask_valid = false;
     cf *p_src = nullptr, *p_u = nullptr, *p_tmp = nullptr;
@@ -290,7 +292,7 @@ struct wust_helmholtz {
             p_tmp = dalloc<cf>((int64_t)op.npad * Rpad);
         } else {
             s32 = dalloc<cf>(cnt);
-            s32_zero_elems = 0;
+            s32_zero_elems = s32_points_elems = 0;
             d32 = dalloc<cf>(cnt);
             if (op.mode == WUST_MODE_REFINED) {
                 d32b = dalloc<cf>(cnt);
@@ -336,18 +338,20 @@ struct wust_helmholtz {
         const char *e = std::getenv("WUST_SKIP_ZERO_COLUMNS");
         if (!op.colflags || !bmask || (e && e[0] == '0')) {
             demote(b, s32, cnt, s);
-            s32_zero_elems = 0;
+            s32_zero_elems = s32_points_elems = 0;
             return;
         }
         WUST_HIP_CHECK(hipMemsetAsync(op.colflags, 0, sizeof(int) * op.m, s));
         if (px) {
             // s32 stays zero between point right-hand sides (their nodes are cleared after the solve),
-            // so only its first use, or a dense use before, needs the full clear
+            // so only its first use, or a dense use before, needs the full clear; until the nodes are
+            // cleared it counts as not zero (an error in between leaves the next use a full clear)
             sparse_demote_flag(b, s32, cnt, px, py, npts, op.npad, Rpad, op.colflags, bmask, s, s32_zero_elems >= cnt);
-            s32_zero_elems = cnt;
+            s32_zero_elems = 0;
+            s32_points_elems = cnt;
         } else {
             demote_flag(b, s32, cnt, (int64_t)op.npad * Rpad, op.colflags, bmask, s);
-            s32_zero_elems = 0;
+            s32_zero_elems = s32_points_elems = 0;
         }
         bmask_valid = true;
         if (px && hpx) {
@@ -432,7 +436,11 @@ struct wust_helmholtz {
         int lead = 0, trail = 0;
         demote_rhs(b, cnt, Rpad, s, lead, trail, px, py, npts, hpx);
         solve_engine(op, s32, d32, b32, Rpad, adjoint, s, lead, trail);
-        if (px && s32_zero_elems >= cnt) launch_clear_points32(s32, px, py, npts, op.npad, Rpad, s);
+        if (px && s32_points_elems > 0) {
+            launch_clear_points32(s32, px, py, npts, op.npad, Rpad, s);
+            s32_zero_elems = s32_points_elems;
+            s32_points_elems = 0;
+        }
         if (op.mode == WUST_MODE_C64) {
             promote(d32, out, cnt, s);
             if (want_residual) st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr, nullptr);
