@@ -178,6 +178,16 @@ constexpr int SWEEP_RB = 1;   // 16-row tiles: 2 * 36 * 16 = 1152 WGs at 560^2 x
 // neighbour rewrote them.
 constexpr int FUSED_OWN = 14;   // owned rows of a 16-row tile
 
+// Stores of a sweep step's outputs (its result block and the next step's operand block) are
+// non-temporal: no step reads them back through this XCD's L2 (the next step's workgroups on the
+// other XCDs see them only after the kernel boundary's write-back, the back substitution and the
+// residual much later), so they go out without holding L2 lines that the boundary must flush.
+// 64.5-64.7 against 65.5-65.9 ms per evaluation at 560^2 x 256, same box.
+__device__ __forceinline__ void store_stream(cf *p, cf v) {
+    typedef float v2f_st __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(v2f_st{v.re, v.im}, reinterpret_cast<v2f_st *>(p));
+}
+
 struct FusedTask {
     const cf *X;              // inverse page
     const cf *B;              // operand block [npad][Rpad]
@@ -293,7 +303,7 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
     for (int q = 0; q < TPW; ++q) {
         const int64_t off = (int64_t)k * Rpad + (ng * TPW + q) * 16 + cc;
         const cf v = tile[q][t][cc];
-        tk.out[off] = v;
+        store_stream(tk.out + off, v);
         if (!tk.Bn) continue;
         const cf vn = (k >= 1) ? tile[q][t - 1][cc] : mk(0.f, 0.f);        // row k - 1 (in the tile when k >= 1)
         const cf vs = (k + 1 < npad) ? tile[q][t + 1][cc] : mk(0.f, 0.f);  // row k + 1 (in the tile when < npad)
@@ -307,7 +317,7 @@ __device__ __forceinline__ void sweep_fused_body(const FusedParams &p) {
             if (k + 1 < npad) cfma(a, ps, vs);
             if (k >= 1) cfma(a, pn, vn);
         }
-        tk.Bn[off] = psrc[q] - a;
+        store_stream(tk.Bn + off, psrc[q] - a);
     }
 }
 
@@ -554,11 +564,11 @@ __device__ __forceinline__ void sweep_fused_body_lds(const FusedParams &p) {
     if (!own) return;
     const int64_t off = (int64_t)k * Rpad + j0 + cc;
     const cf v = tile[t][cc];
-    tk.out[off] = v;
+    store_stream(tk.out + off, v);
     if (!tk.Bn) return;
     const cf vn = (k >= 1) ? tile[t - 1][cc] : mk(0.f, 0.f);
     const cf vs = (k + 1 < npad) ? tile[t + 1][cc] : mk(0.f, 0.f);
-    tk.Bn[off] = ep.next(v, vn, vs);
+    store_stream(tk.Bn + off, ep.next(v, vn, vs));
 }
 
 // s_waitcnt immediate that waits until at most n vector memory operations are outstanding
@@ -761,11 +771,11 @@ __device__ __forceinline__ void sweep_fused_body_dma(const FusedParams &p) {
     if (!own) return;
     const int64_t off = (int64_t)k * Rpad + j0 + cc;
     const cf v = tile[t][cc];
-    tk.out[off] = v;
+    store_stream(tk.out + off, v);
     if (!tk.Bn) return;
     const cf vn = (k >= 1) ? tile[t - 1][cc] : mk(0.f, 0.f);
     const cf vs = (k + 1 < npad) ? tile[t + 1][cc] : mk(0.f, 0.f);
-    tk.Bn[off] = ep.next(v, vn, vs);
+    store_stream(tk.Bn + off, ep.next(v, vn, vs));
 }
 
 template <int ALAY, int DEPTH, bool LATE, bool BUF>
