@@ -343,8 +343,12 @@ struct StepEpilogue {
         return __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(ptr), (short)0, (use && ptr) ? (int)(elems * 8) : 0,
                                                  0x00020000);
     }
+    // aux 0: cached; aux 2: non-temporal (the base and next-source blocks are read once per launch:
+    // kept out of L2, they leave it to the pages and operand blocks that the strips share; 1.2-1.6 ms
+    // per evaluation at 560^2 x 256, same box)
+    template <int AUX = 0>
     __device__ __forceinline__ static cf ld(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-        const v2f_t u = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+        const v2f_t u = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, AUX));
         return mk(u.x, u.y);
     }
     __device__ __forceinline__ StepEpilogue(const FusedTask &tk, int npad_, int Rpad, int i0, int j0, int k_, int cc,
@@ -368,13 +372,13 @@ struct StepEpilogue {
     __device__ __forceinline__ void load(int g, int c, int Rpad) {
         if (use_base) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) pbase[r] = ld(rbase, obase + (uint32_t)(((4 * g + r) * Rpad + c) * 8));
+            for (int r = 0; r < 4; ++r) pbase[r] = ld<2>(rbase, obase + (uint32_t)(((4 * g + r) * Rpad + c) * 8));
         }
         if (use_next) {
             pc = ld(rcc, (uint32_t)k * 8);
             pa = ld(rca, (uint32_t)ka * 8);
             pz = ld(rcz, (uint32_t)kz * 8);
-            psrc = ld(rsrc, osrc);
+            psrc = ld<2>(rsrc, osrc);
         }
     }
     // B' = src' - P v for output element v with row neighbours vn (k - 1) and vs (k + 1)
