@@ -181,6 +181,26 @@ def test_ring160_curvature(ring160):
     assert abs(kg - ko) / ko <= 1e-8
 
 
+def test_unusable_source_scale(small):
+    """A transmitter with no valid receiver leaves its source-scale denominator at zero, which the
+    reference rejects (Runtime/matlab/wust_oracle.m:55-57: WUST:NumericalFailure, 'Unusable
+    source-scale denominator/numerator').  The engine reads the fits' failure flags back once, after
+    the last source batch, so the error must come whichever batch held the transmitter, and the
+    same observations with every receiver valid must still evaluate."""
+    obs = make_obs(small)
+    init = 1500 * np.ones((small.ny, small.nx))
+    for tx in (0, small.n_tx - 1):
+        mask = obs["mask"].copy()
+        mask[tx, :, 0] = False
+        for batch in (1, 2, small.n_tx):
+            with pytest.raises(WustError) as ei:
+                G.wust_oracle(init, dict(obs, mask=mask), 0, dict(small.cfg, source_batch_size=batch))
+            assert ei.value.identifier == "WUST:NumericalFailure", (tx, batch)
+            assert "source-scale" in str(ei.value), (tx, batch)
+    loss, g = G.wust_oracle(init, obs, 0, dict(small.cfg, source_batch_size=1))
+    assert np.isfinite(loss) and bool(torch.all(torch.isfinite(g)))
+
+
 def test_error_paths(small):
     obs = make_obs(small)
     init = 1500 * np.ones((small.ny, small.nx))
