@@ -104,3 +104,52 @@ def test_pivoted_refactorization(monkeypatch, ny, nx, factor):
         Hm = ref.H.conj().T if adjoint else ref.H
         res = (Hm @ u.reshape(-1, R, order="F") - bf).reshape(ny, nx, R, order="F")[interior]
         assert np.linalg.norm(res) <= 1e-12 * np.linalg.norm(bf), adjoint
+
+
+def test_field_shapes_checked():
+    """Fields whose leading dimensions are not the operator's grid are WUST:InvalidInput before any
+    launch (the engine reads ny * nx * R values from them; Runtime/solver/applyBlockLU.cu:380-388).
+    Among them a source given as [R, ny, nx] instead of [ny, nx, R]."""
+    ny, nx = 40, 23
+    x, y, c = _grid(ny, nx)
+    s = G.HelmholtzSolver(x, y, c, None, 2.2e5, -1, 10.0, 2e-3, OPTS)
+    good = _rhs(ny, nx, 3, 2)
+    for bad in (np.transpose(good, (2, 0, 1)), good[:-1], good[:, :-1], good[:, :, :0], good[..., None]):
+        with pytest.raises(WustError) as ei:
+            s.solve(bad, False)
+        assert ei.value.identifier == "WUST:InvalidInput"
+        with pytest.raises(WustError) as ei:
+            s.massApply(bad)
+        assert ei.value.identifier == "WUST:InvalidInput"
+    # the handle is untouched: a valid solve afterwards meets the tolerance
+    u, _, st = s.solve(good, False, return_stats=True)
+    assert u.shape == good.shape and st["final_residual"] <= 1e-10
+
+
+def test_oracle_shapes_checked():
+    """Observation slices and curvature inputs that disagree with the transmitter / receiver counts or
+    the grid are WUST:InvalidInput."""
+    ny, nx = 40, 23
+    x, y, c = _grid(ny, nx)
+    tx = np.array([2 * ny + 5, 10 * ny + 7, 15 * ny + 30], np.int64)
+    rx = np.array([3 * ny + 4, 12 * ny + 20, 18 * ny + 9, 20 * ny + 33], np.int64)
+    rng = np.random.default_rng(3)
+    Y = rng.standard_normal((3, 4, 1)) + 1j * rng.standard_normal((3, 4, 1))
+    obs = dict(shape_yx=(ny, nx), x_m=x, y_m=y, tx_index=tx, rx_index=rx, frequencies_hz=[2.2e5],
+               Y=Y, mask=np.ones((3, 4, 1), bool))
+    cfg = dict(backend="gpu", stencil_bounds=[1400, 1700], pml_strength=10.0, pml_m=2e-3, wavenumber="continuum",
+               source_batch_size=3)
+    for key, bad in (("Y", Y[:, :3]), ("Y", np.transpose(Y, (1, 0, 2))), ("mask", np.ones((4, 3, 1), bool))):
+        with pytest.raises(WustError) as ei:
+            G.wust_oracle(c, dict(obs, **{key: bad}), 0, cfg)
+        assert ei.value.identifier == "WUST:InvalidInput"
+    loss, g, state = G.wust_oracle(c, obs, 0, cfg, nargout=3)
+    assert np.isfinite(loss) and g.shape == (ny, nx)
+    for bad_dir in (np.zeros((nx, ny)), np.zeros((ny, nx - 1))):
+        with pytest.raises(WustError) as ei:
+            G.directional_curvature(state, bad_dir)
+        assert ei.value.identifier == "WUST:InvalidInput"
+    with pytest.raises(WustError) as ei:
+        G.directional_curvature(state, np.zeros((ny, nx)), receiver_mask=np.ones((4, 3), bool))
+    assert ei.value.identifier == "WUST:InvalidInput"
+    assert np.isfinite(G.directional_curvature(state, np.ones((ny, nx))))

@@ -53,14 +53,23 @@ def _grid_to_dev(a, device, dtype=torch.float64):
     return t.t().contiguous()
 
 
-def _field_to_dev(a, device):
-    """[Ny, Nx, R] (or [Ny, Nx]) complex -> contiguous complex128 (R, Nx, Ny)."""
+def _field_to_dev(a, device, ny, nx, message):
+    """[Ny, Nx, R] (or [Ny, Nx]) complex -> contiguous complex128 (R, Nx, Ny).
+
+    The engine reads Ny * Nx * R values from the result, so fields whose leading
+    dimensions are not the operator's grid are rejected here with ``message``
+    (Runtime/solver/applyBlockLU.cu:380-388 rejects them as WUST:InvalidInput)."""
     t = torch.as_tensor(a, device=device)
     if t.dim() == 2:
         t = t.unsqueeze(-1)
-    if t.dim() != 3:
-        raise WustError("WUST:InvalidInput", "sources must be [ny, nx] or [ny, nx, n_source]")
+    if t.dim() != 3 or tuple(t.shape[:2]) != (ny, nx) or t.shape[2] < 1:
+        raise WustError("WUST:InvalidInput", message)
     return t.to(torch.complex128).permute(2, 1, 0).contiguous()
+
+
+def _check_shape(a, shape, message):
+    if tuple(a.shape) != tuple(shape):
+        raise WustError("WUST:InvalidInput", message)
 
 
 def _field_from_dev(t, squeeze):
@@ -207,7 +216,7 @@ class HelmholtzSolver:
     def solve(self, src, adjoint=False, tol=None, max_iter=None, return_stats=False):
         """[u, v] = solve(src, adjoint): H u = src or H^H u = src; v = (C q') u."""
         squeeze = torch.as_tensor(src).dim() == 2
-        s = _field_to_dev(src, self.device)
+        s = _field_to_dev(src, self.device, self.ny, self.nx, "Inconsistent Block-LU input shapes.")
         R = s.shape[0]
         out = torch.empty_like(s)
         st = N.SolveStats()
@@ -229,7 +238,7 @@ class HelmholtzSolver:
     def massApply(self, v, adjoint=False):
         """Exact mass-stencil derivative (output border zeroed; adjoint zeroes the input border)."""
         squeeze = torch.as_tensor(v).dim() == 2
-        s = _field_to_dev(v, self.device)
+        s = _field_to_dev(v, self.device, self.ny, self.nx, "massApply expects an [ny, nx] or [ny, nx, n] field")
         out = torch.empty_like(s)
         with torch.cuda.device(self.device):
             N.check(self._lib.wust_helmholtz_mass_apply(
@@ -261,6 +270,12 @@ def _obs_slice(obs, fi, device):
     return Yd, Md
 
 
+def _check_obs_slice(Yd, Md, ntx, nrx):
+    # the engine reads n_tx * n_rx observations and mask entries
+    _check_shape(Yd, (nrx, ntx), "obs.Y must be [n_tx, n_rx, n_freq]")
+    _check_shape(Md, (nrx, ntx), "obs.mask must be [n_tx, n_rx, n_freq]")
+
+
 def wust_oracle(c, obs, fi, cfg, nargout=2):
     """Loss / gradient / retained state at frequency index ``fi`` (0-based).
 
@@ -283,6 +298,7 @@ def wust_oracle(c, obs, fi, cfg, nargout=2):
     tx = np.ascontiguousarray(np.asarray(obs["tx_index"], dtype=np.int64).ravel())
     rx = np.ascontiguousarray(np.asarray(obs["rx_index"], dtype=np.int64).ravel())
     Yd, Md = _obs_slice(obs, fi, dev)
+    _check_obs_slice(Yd, Md, tx.size, rx.size)
     grad = torch.empty((nx, ny), dtype=torch.float64, device=dev) if need_gradient else None
     loss = C.c_double()
     t0 = time.perf_counter()
@@ -320,11 +336,14 @@ def directional_curvature(state, direction, rx_index=None, receiver_mask=None):
             raise WustError("WUST:InvalidInput", "rx_index differs from the receivers of the retained oracle state")
     solver = state.solver
     dev = solver.device
+    direction = torch.as_tensor(direction)
+    _check_shape(direction, (solver.ny, solver.nx), "direction must be [ny, nx]")
     d = _grid_to_dev(direction, dev)
     if receiver_mask is None:
         Md = state.extra["receiver_mask"]
     else:
         Md = torch.as_tensor(receiver_mask, device=dev).to(torch.uint8).t().contiguous()
+        _check_shape(Md, tuple(state.extra["receiver_mask"].shape), "receiver_mask must be [n_tx, n_rx]")
     kappa = C.c_double()
     with torch.cuda.device(dev):
         N.check(solver._lib.wust_directional_curvature(
