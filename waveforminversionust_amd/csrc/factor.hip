@@ -71,51 +71,79 @@ struct SchurParams {
     int ntasks, n, npad;
 };
 
-// T = D - sum_t P_t X_t Q_t with tridiagonal P, Q (three-point stencils on X).
+// T = D - sum_t P_t X_t Q_t with tridiagonal P, Q (three-point stencils on X).  A thread takes one
+// row and SCHUR_COLS consecutive columns: the 3 x (SCHUR_COLS + 2) page entries around them are
+// loaded once, from clamped in-range indices (a stencil point that leaves the block has a zero
+// coefficient and adds an exact zero), all issued together, and shared by the columns' stencils; the
+// column coefficients are the same across the wave.  A workgroup covers 64 rows (one per lane) and
+// four waves of column runs.  At 576^2 x 2 chains, runs of two columns took 6.6-7.0 us per launch
+// against 7.6 us for one column per thread in 256-row workgroups (the third row block of 576 then
+// idles three of its four waves), 7.9 us for runs of one, ~6.9 us for three or four and 8.8 us for
+// eight (fewer waves in flight).
+constexpr int SCHUR_COLS = 2;
+constexpr int SCHUR_ROWS = 64;
+dim3 schur_grid(int ld, int ntasks) {
+    return dim3((ld + SCHUR_ROWS - 1) / SCHUR_ROWS, (ld + 4 * SCHUR_COLS - 1) / (4 * SCHUR_COLS), ntasks);
+}
 __global__ void __launch_bounds__(256) schur_kernel(SchurParams p) {
     const SchurTask &tk = p.task[blockIdx.z];
-    int r = blockIdx.x * 256 + threadIdx.x;
-    int c = blockIdx.y;
-    if (tk.clear_norms && blockIdx.x == 0 && c == 0 && threadIdx.x < NS_KMAX) tk.clear_norms[threadIdx.x] = 0.f;
-    if (r >= p.npad) return;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r = blockIdx.x * SCHUR_ROWS + (threadIdx.x & 63);
+    const int c0 = (blockIdx.y * 4 + wave) * SCHUR_COLS;
+    if (tk.clear_norms && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < NS_KMAX) tk.clear_norms[threadIdx.x] = 0.f;
     const int n = p.n, ld = p.npad;
-    cf val = mk(0.f, 0.f);
-    if (r >= n || c >= n) {
-        val = (r == c) ? mk(1.f, 0.f) : mk(0.f, 0.f);
-        tk.T[r + (int64_t)ld * c] = val;
-        return;
-    }
-    if (r == c) val = tk.Dc[r];
-    else if (c == r - 1) val = tk.Dn[r];
-    else if (c == r + 1) val = tk.Ds[r];
-    // the nine X entries come from clamped in-range indices with zero coefficients where the
-    // stencil leaves the block, so all loads issue at once; a zero coefficient adds an exact zero
-    const int rows[3] = {max(r - 1, 0), r, min(r + 1, n - 1)};
-    const int64_t cols[3] = {(int64_t)ld * c, (int64_t)ld * max(c - 1, 0), (int64_t)ld * min(c + 1, n - 1)};
+    if (r >= ld || c0 >= ld) return;
     const cf zero = mk(0.f, 0.f);
-    auto term = [&](const SchurTerm &st) {
-        const cf qm = (c >= 1) ? st.Qs[c - 1] : zero;
-        const cf q0 = st.Qc[c];
-        const cf qp = (c + 1 < n) ? st.Qn[c + 1] : zero;
-        const cf pr[3] = {(r >= 1) ? st.Pn[r] : zero, st.Pc[r], (r + 1 < n) ? st.Ps[r] : zero};
-        cf x[3][3];
+    cf val[SCHUR_COLS];
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int q = 0; q < 3; ++q) x[a][q] = st.X[rows[a] + cols[q]];
-        cf acc = zero;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            cf xq = q0 * x[a][0];
-            cfma(xq, qm, x[a][1]);
-            cfma(xq, qp, x[a][2]);
-            cfma(acc, pr[a], xq);
+    for (int i = 0; i < SCHUR_COLS; ++i) {
+        const int c = c0 + i;
+        val[i] = zero;
+        if (r < n && c < n) {
+            if (r == c) val[i] = tk.Dc[r];
+            else if (c == r - 1) val[i] = tk.Dn[r];
+            else if (c == r + 1) val[i] = tk.Ds[r];
         }
-        val = val - acc;
-    };
-    if (tk.nterms >= 1) term(tk.t[0]);
-    if (tk.nterms >= 2) term(tk.t[1]);
-    tk.T[r + (int64_t)ld * c] = val;
+    }
+    if (r < n) {
+        const int rows[3] = {max(r - 1, 0), r, min(r + 1, n - 1)};
+        auto term = [&](const SchurTerm &st) {
+            const cf pr[3] = {(r >= 1) ? st.Pn[r] : zero, st.Pc[r], (r + 1 < n) ? st.Ps[r] : zero};
+            cf x[3][SCHUR_COLS + 2];   // columns c0 - 1 .. c0 + SCHUR_COLS, clamped to the block
+#pragma unroll
+            for (int q = 0; q < SCHUR_COLS + 2; ++q) {
+                const int64_t col = (int64_t)ld * min(max(c0 - 1 + q, 0), n - 1);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) x[a][q] = st.X[rows[a] + col];
+            }
+#pragma unroll
+            for (int i = 0; i < SCHUR_COLS; ++i) {
+                const int c = c0 + i;
+                if (c >= n) continue;
+                const cf qm = (c >= 1) ? st.Qs[c - 1] : zero;
+                const cf q0 = st.Qc[c];
+                const cf qp = (c + 1 < n) ? st.Qn[c + 1] : zero;
+                cf acc = zero;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    cf xq = q0 * x[a][i + 1];
+                    cfma(xq, qm, x[a][i]);
+                    cfma(xq, qp, x[a][i + 2]);
+                    cfma(acc, pr[a], xq);
+                }
+                val[i] = val[i] - acc;
+            }
+        };
+        if (tk.nterms >= 1) term(tk.t[0]);
+        if (tk.nterms >= 2) term(tk.t[1]);
+    }
+#pragma unroll
+    for (int i = 0; i < SCHUR_COLS; ++i) {
+        const int c = c0 + i;
+        if (c >= ld) break;
+        // rows and columns beyond the block carry the identity
+        tk.T[r + (int64_t)ld * c] = (r >= n || c >= n) ? ((r == c) ? mk(1.f, 0.f) : zero) : val[i];
+    }
 }
 
 // X = I over the whole padded page (the pivoted path's getrs then overwrites the n x n block; the
@@ -627,8 +655,7 @@ void launch_schur_left(const Operator &op, int j, cf *T, hipStream_t s) {
     tk.nterms = 0;
     tk.T = T;
     if (j >= 1) add_term(op, tk.t[tk.nterms++], op.page(j - 1), j, Ldirs, j - 1, Udirs);
-    dim3 grid((op.npad + 255) / 256, op.npad, 1);
-    launch_k(PC_SCHUR, schur_kernel, grid, dim3(256), 0, s, sp);
+    launch_k(PC_SCHUR, schur_kernel, schur_grid(op.npad, 1), dim3(256), 0, s, sp);
 }
 
 namespace {
@@ -799,8 +826,7 @@ struct FactorRun {
             if ((kind == 1 || kind == 2) && j + 1 <= m - 1)  // - U_j X^R_{j+1} L_{j+1}
                 add_term(op, tk.t[tk.nterms++], op.page(j + 1), j, Udirs, j + 1, Ldirs);
         }
-        dim3 grid((ld + 255) / 256, ld, sp.ntasks);
-        launch_k(PC_SCHUR, schur_kernel, grid, dim3(256), 0, s, sp);
+        launch_k(PC_SCHUR, schur_kernel, schur_grid(ld, sp.ntasks), dim3(256), 0, s, sp);
 
         if (pivoted) {
             // the reference's inversion: partial-pivot LU of T, then X = T^-1 by getrs against I
