@@ -167,7 +167,8 @@ void stencil_params(double vmin, double vmax, double f, double h, double g, doub
 }
 
 struct Retained {
-    cd *u = nullptr;       // forward field (solve layout c128)
+    cd *buf = nullptr;     // storage of the forward field (c128-sized)
+    FieldView u;           // the forward field in buf
     cd *scale = nullptr;   // device [Rpad]
     int first = 0, nb = 0, Rpad = 0;
 };
@@ -182,7 +183,7 @@ struct wust_helmholtz {
     // leading elements of b64 known to hold zeros (the oracle evaluation undoes what it scatters); the
     // solve layout of a batch of Rpad right-hand sides is the first field_elems(Rpad) elements
     int64_t b64_zero_elems = 0;
-    cf *s32 = nullptr, *d32 = nullptr, *d32b = nullptr, *b32 = nullptr;
+    cf *s32 = nullptr, *d32 = nullptr, *b32 = nullptr;   // d32: the corrections after the first (refined)
     cf *r32 = nullptr;          // refinement residuals, the correction solves' right-hand sides
     int64_t s32_zero_elems = 0; // leading elements of s32 known to be zero (point right-hand sides
                                 // write their nodes into it and clear them after their solve)
@@ -245,14 +246,13 @@ struct wust_helmholtz {
         dpool(lam64, cnt);
         if (s32) dpool(s32, cnt);
         if (d32) dpool(d32, cnt);
-        if (d32b) dpool(d32b, cnt);
         if (r32) dpool(r32, cnt);
         if (bmask) dpool(bmask, cnt / 64);
         if (b32) dpool(b32, solve_work_elems(op, ws_Rpad));
     }
     void release_retained() {
         for (auto &r : retained) {
-            dfree(r.u);
+            dfree(r.buf);
             dfree(r.scale);
         }
         retained.clear();
@@ -264,7 +264,6 @@ struct wust_helmholtz {
         dfree(lam64);
         dfree(s32);
         dfree(d32);
-        dfree(d32b);
         dfree(r32);
         dfree(bmask);
         dfree(b32);
@@ -293,9 +292,8 @@ struct wust_helmholtz {
         } else {
             s32 = dalloc<cf>(cnt);
             s32_zero_elems = s32_points_elems = 0;
-            d32 = dalloc<cf>(cnt);
             if (op.mode == WUST_MODE_REFINED) {
-                d32b = dalloc<cf>(cnt);
+                d32 = dalloc<cf>(cnt);
                 r32 = dalloc<cf>(cnt);
             }
             bmask = dalloc<uint8_t>(cnt / 64);
@@ -374,10 +372,14 @@ struct wust_helmholtz {
 
     // Solve in the solve layout: b (c128, Rpad) -> out (c128).  b is not modified.  px / py (npts
     // nodes, or null): every nonzero of b lies at one of these nodes.
-    wust_solve_stats solve_internal(const cd *b, cd *out, int R, int Rpad, bool adjoint, double tol, int max_iter,
-                                    bool want_residual, hipStream_t s, const int *px = nullptr,
+    // The solution lands in `out` (c128-sized) and `res` says how: the FP64 field, the c64 iterate in
+    // out's first half (c64 mode, or refined with no correction needed), or the pair (iterate,
+    // correction) in its two halves (refined with one correction, the usual case).
+    wust_solve_stats solve_internal(const cd *b, cd *out, FieldView &res, int R, int Rpad, bool adjoint, double tol,
+                                    int max_iter, bool want_residual, hipStream_t s, const int *px = nullptr,
                                     const int *py = nullptr, int npts = 0, const int *hpx = nullptr) {
         wust_solve_stats st{};
+        res = FieldView{};
         const int64_t cnt = field_elems(Rpad);
         double *nb2 = norms, *nr2 = norms + Rpad;
         bmask_valid = false;   // set by demote_rhs for this b
@@ -428,48 +430,56 @@ struct wust_helmholtz {
             solve64_to_plain32(b, p_src, op.m, op.npad, R, Rpad, s);
             solve_plain(op, plain, p_src, p_u, p_tmp, R, adjoint, s);
             plain32_to_solve64(p_u, out, op.m, op.npad, R, Rpad, s);
+            res.u64 = out;
             if (want_residual) {
                 st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr, nullptr);
             }
             return st;
         }
         int lead = 0, trail = 0;
+        cf *d0 = reinterpret_cast<cf *>(out), *d1 = d0 + cnt;   // out's two c64 halves
         demote_rhs(b, cnt, Rpad, s, lead, trail, px, py, npts, hpx);
-        solve_engine(op, s32, d32, b32, Rpad, adjoint, s, lead, trail);
+        solve_engine(op, s32, d0, b32, Rpad, adjoint, s, lead, trail);
         if (px && s32_points_elems > 0) {
             launch_clear_points32(s32, px, py, npts, op.npad, Rpad, s);
             s32_zero_elems = s32_points_elems;
             s32_points_elems = 0;
         }
+        res.a = d0;
         if (op.mode == WUST_MODE_C64) {
-            promote(d32, out, cnt, s);
-            if (want_residual) st.initial_residual = st.final_residual = residual_pass(out, nullptr, nullptr, nullptr, nullptr);
+            if (want_residual) st.initial_residual = st.final_residual = residual_pass(nullptr, nullptr, d0, nullptr, nullptr);
             return st;
         }
-        // refinement: U = D_1 + D_2 + ...; R = b - H U; D_{k+1} = solve(R).  The first pass keeps
-        // U = D_1 in single precision (d32, not stored in FP64) and the first correction goes to a
-        // second buffer (d32b), so the second pass reads two c64 fields instead of an FP64 one;
-        // later passes accumulate in FP64, alternating between two buffers ordered so that the
-        // usual single correction ends in `out` without a copy.
-        cd *bufs[2] = {t64 == out ? u64 : t64, out};
-        double rel = residual_pass(nullptr, nullptr, d32, nullptr, r32);
+        // refinement: U = D_1 + D_2 + ...; R = b - H U; D_{k+1} = solve(R).  D_1 and D_2 stay in single
+        // precision in out's halves, and the pass after the first correction only measures the residual
+        // of D_1 + D_2: with it below tol (the usual case) the result is that pair, and neither the FP64
+        // sum nor the residual is stored.  Otherwise the pass runs again to store both (the same
+        // values), and later passes accumulate in FP64, alternating between two buffers ordered so
+        // that the result ends in `out`.
+        double rel = residual_pass(nullptr, nullptr, d0, nullptr, r32);
         st.initial_residual = st.final_residual = rel;
-        int it = 0, cur = 1;
-        if (!(rel > tol) || max_iter <= 0) {
-            promote(d32, out, cnt, s);
-        } else {
-            solve_engine(op, r32, d32b, b32, Rpad, adjoint, s);
+        int it = 0;
+        if (rel > tol && max_iter > 0) {
+            solve_engine(op, r32, d1, b32, Rpad, adjoint, s);
             it = 1;
-            rel = st.final_residual = residual_pass(nullptr, d32, d32b, bufs[cur], r32);
-            while (rel > tol && it < max_iter) {
-                solve_engine(op, r32, d32, b32, Rpad, adjoint, s);
-                ++it;
-                const cd *u_old = bufs[cur];
-                cur ^= 1;
-                rel = st.final_residual = residual_pass(u_old, nullptr, d32, bufs[cur], r32);
+            res.b = d1;
+            rel = st.final_residual = residual_pass(nullptr, d0, d1, nullptr, nullptr);
+            if (rel > tol && it < max_iter) {
+                cd *bufs[2] = {t64 == out ? u64 : t64, out};
+                int cur = 0;
+                rel = residual_pass(nullptr, d0, d1, bufs[cur], r32);
+                while (rel > tol && it < max_iter) {
+                    solve_engine(op, r32, d32, b32, Rpad, adjoint, s);
+                    ++it;
+                    const cd *u_old = bufs[cur];
+                    cur ^= 1;
+                    rel = st.final_residual = residual_pass(u_old, nullptr, d32, bufs[cur], r32);
+                }
+                if (bufs[cur] != out)
+                    WUST_HIP_CHECK(hipMemcpyAsync(out, bufs[cur], sizeof(cd) * cnt, hipMemcpyDeviceToDevice, s));
+                res = FieldView{};
+                res.u64 = out;
             }
-            if (bufs[cur] != out)
-                WUST_HIP_CHECK(hipMemcpyAsync(out, bufs[cur], sizeof(cd) * cnt, hipMemcpyDeviceToDevice, s));
         }
         st.iterations = it;
         if (!std::isfinite(st.final_residual))
@@ -711,9 +721,10 @@ int wust_helmholtz_solve(wust_helmholtz *h, const void *d_src, void *d_out, int6
         h->ensure_ws(Rpad);
         h->b64_zero_elems = 0;
         ext_to_solve64((const cd *)d_src, h->b64, h->op.ny, h->op.nx, h->op.npad, (int)R, Rpad, nullptr, s);
+        FieldView u;
         wust_solve_stats st =
-            h->solve_internal(h->b64, h->u64, (int)R, Rpad, adjoint != 0, tol, max_iter, stats != nullptr, s);
-        solve64_to_ext(h->u64, (cd *)d_out, h->op.ny, h->op.nx, h->op.npad, (int)R, Rpad, s);
+            h->solve_internal(h->b64, h->u64, u, (int)R, Rpad, adjoint != 0, tol, max_iter, stats != nullptr, s);
+        solve_to_ext(u, (cd *)d_out, h->op.ny, h->op.nx, h->op.npad, (int)R, Rpad, s);
         WUST_HIP_CHECK(hipEventRecord(e1, s));
         WUST_HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;
@@ -796,13 +807,14 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
             cd *ufwd = h->u64;
             if (retain) ufwd = dalloc<cd>(cnt);
             try {
-                h->solve_internal(h->b64, ufwd, nb, Rpad, false, tol, max_iter, false, s, txx + first, txy + first,
-                                  nb, tx_x.data() + first);
+                FieldView fwd;
+                h->solve_internal(h->b64, ufwd, fwd, nb, Rpad, false, tol, max_iter, false, s, txx + first,
+                                  txy + first, nb, tx_x.data() + first);
                 // source fit, residual, loss and the adjoint source
                 cd *adj = h->b64;  // forward RHS no longer needed: undo its impulses
                 launch_impulses(adj, txx + first, txy + first, nb, op.npad, Rpad, s, 0.0);
                 SourceFitArgs fa{};
-                fa.u = ufwd;
+                fa.u = fwd;
                 fa.Y = (const cd *)d_Y;
                 fa.mask = d_mask;
                 fa.rx_x = rxx;
@@ -824,11 +836,12 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
                 WUST_HIP_CHECK(hipMemcpyAsync(h->scales_dev + first, h->scale, sizeof(cd) * nb,
                                               hipMemcpyDeviceToDevice, s));
                 if (need_gradient) {
-                    h->solve_internal(adj, h->lam64, nb, Rpad, true, tol, max_iter, false, s, rxx, rxy, (int)n_rx,
-                                      h->rx_x.data());
+                    FieldView lam;
+                    h->solve_internal(adj, h->lam64, lam, nb, Rpad, true, tol, max_iter, false, s, rxx, rxy,
+                                      (int)n_rx, h->rx_x.data());
                     GradientArgs ga{};
-                    ga.u = ufwd;
-                    ga.lam = h->lam64;
+                    ga.u = fwd;
+                    ga.lam = lam;
                     ga.vsm = op.vsm;
                     ga.scale = h->scale;
                     ga.grad = (double *)d_grad;
@@ -847,7 +860,8 @@ int wust_oracle_eval(wust_helmholtz *h, const int64_t *tx_index, int64_t n_tx, c
                 h->b64_zero_elems = zero_after;
                 if (retain) {
                     Retained rt;
-                    rt.u = ufwd;
+                    rt.buf = ufwd;
+                    rt.u = fwd;
                     rt.scale = dalloc<cd>(Rpad);
                     WUST_HIP_CHECK(hipMemsetAsync(rt.scale, 0, sizeof(cd) * Rpad, s));
                     WUST_HIP_CHECK(hipMemcpyAsync(rt.scale, h->scale, sizeof(cd) * nb, hipMemcpyDeviceToDevice, s));
@@ -912,9 +926,10 @@ int wust_directional_curvature(wust_helmholtz *h, const double *d_direction, con
             ca.npad = op.npad;
             ca.Rpad = rt.Rpad;
             launch_curvature_source(ca, s);
-            h->solve_internal(h->b64, h->lam64, rt.nb, rt.Rpad, false, tol, max_iter, false, s);
+            FieldView w;
+            h->solve_internal(h->b64, h->lam64, w, rt.nb, rt.Rpad, false, tol, max_iter, false, s);
             CurvGatherArgs ga{};
-            ga.u = h->lam64;
+            ga.u = w;
             ga.scale = rt.scale;
             ga.mask = d_mask;
             ga.rx_x = rxx;

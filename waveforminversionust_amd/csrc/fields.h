@@ -3,8 +3,35 @@
 
 namespace wust {
 
-__host__ __device__ __forceinline__ cd tod_any(cd v) { return v; }
-__host__ __device__ __forceinline__ cd tod_any(cf v) { return tod(v); }
+// A field in the solve layout: the FP64 field u64, or the FP64 sum of two c64 fields a + b (b null:
+// a alone).  A refined solve with a single correction leaves its result as the pair (first iterate,
+// correction), so that no FP64 copy of it is written; the sum is formed where it is read, by the same
+// FP64 addition that storing it would have used.
+// load() and value() split one access, so that a load issued ahead of its use stays in flight until
+// the use (the sum waits for both loads).
+struct FieldRaw {
+    cd u;
+    cf a, b;
+};
+struct FieldView {
+    const cd *u64 = nullptr;
+    const cf *a = nullptr, *b = nullptr;
+    __host__ __device__ __forceinline__ FieldRaw load(int64_t i) const {
+        FieldRaw r{};
+        if (u64) {
+            r.u = u64[i];
+        } else {
+            r.a = a[i];
+            if (b) r.b = b[i];
+        }
+        return r;
+    }
+    __host__ __device__ __forceinline__ cd value(const FieldRaw &r) const {
+        if (u64) return r.u;
+        return b ? tod(r.a) + tod(r.b) : tod(r.a);
+    }
+    __host__ __device__ __forceinline__ cd at(int64_t i) const { return value(load(i)); }
+};
 
 struct ResidualArgs {
     const cd *w64;        // [9][m][n]
@@ -23,7 +50,7 @@ struct ResidualArgs {
 };
 
 struct SourceFitArgs {
-    const cd *u;          // forward field, solve layout
+    FieldView u;          // forward field
     const cd *Y;          // [ntx, nrx] column-major
     const uint8_t *mask;  // [ntx, nrx]
     const int *rx_x, *rx_y;  // interior coordinates, -1 for receivers on the ring
@@ -35,7 +62,7 @@ struct SourceFitArgs {
 };
 
 struct GradientArgs {
-    const cd *u, *lam;    // solve layout c128
+    FieldView u, lam;     // forward and adjoint fields
     const cd *vsm;        // [Ny*Nx]
     const cd *scale;      // [R]
     double *grad;         // [Ny*Nx], accumulated
@@ -45,7 +72,7 @@ struct GradientArgs {
 };
 
 struct CurvSrcArgs {
-    const cd *u;          // retained forward field, solve layout
+    FieldView u;          // retained forward field
     const cd *vsm;
     const double *direction;  // [Ny*Nx]
     cd *src;              // solve layout c128
@@ -54,7 +81,7 @@ struct CurvSrcArgs {
 };
 
 struct CurvGatherArgs {
-    const cd *u;
+    FieldView u;          // the curvature solve's field
     const cd *scale;
     const uint8_t *mask;
     const int *rx_x, *rx_y;
@@ -63,10 +90,8 @@ struct CurvGatherArgs {
 };
 
 void ext_to_solve64(const cd *ext, cd *out, int ny, int nx, int npad, int R, int Rpad, double *norm2, hipStream_t s);
-void solve64_to_ext(const cd *in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s);
-void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s);
+void solve_to_ext(const FieldView &in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s);
 void launch_residual(const ResidualArgs &a, hipStream_t s);
-void promote(const cf *in, cd *out, int64_t count, hipStream_t s);
 void demote(const cd *in, cf *out, int64_t count, hipStream_t s);
 // demote, and flags[i / per] = 1 for every block of `per` consecutive values that holds a nonzero
 // (flags pre-zeroed; per a multiple of 64); mask[i / 64] = whether values 64 (i / 64) .. + 63 hold

@@ -60,9 +60,8 @@ __global__ void __launch_bounds__(256) ext_to_solve_kernel(const cd *__restrict_
     }
 }
 
-template <typename TI>
-__global__ void __launch_bounds__(256) solve_to_ext_kernel(const TI *__restrict__ in, cd *__restrict__ ext, int ny,
-                                                           int nx, int n, int npad, int R, int Rpad) {
+__global__ void __launch_bounds__(256) solve_to_ext_kernel(FieldView in, cd *__restrict__ ext, int ny, int nx, int n,
+                                                           int npad, int R, int Rpad) {
     __shared__ cd tile[32][33];
     const int x = blockIdx.z;  // full grid column 0..nx-1
     const int y0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
@@ -71,7 +70,7 @@ __global__ void __launch_bounds__(256) solve_to_ext_kernel(const TI *__restrict_
     for (int k = ty; k < 32; k += 8) {
         int y = y0 + k, r = r0 + tx;
         cd v = mkd(0, 0);
-        if (xin && y >= 1 && y <= ny - 2 && r < R) v = tod_any(in[((int64_t)(x - 1) * npad + (y - 1)) * Rpad + r]);
+        if (xin && y >= 1 && y <= ny - 2 && r < R) v = in.at(((int64_t)(x - 1) * npad + (y - 1)) * Rpad + r);
         tile[k][tx] = v;
     }
     __syncthreads();
@@ -246,10 +245,6 @@ __global__ void __launch_bounds__(256) residual_kernel(ResidualArgs a) {
     }
 }
 
-__global__ void promote_kernel(const cf *__restrict__ in, cd *__restrict__ out, int64_t count) {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < count) out[i] = tod(in[i]);
-}
 __global__ void demote_kernel(const cd *__restrict__ in, cf *__restrict__ out, int64_t count) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < count) out[i] = tof(in[i]);
@@ -343,7 +338,7 @@ __global__ void __launch_bounds__(256) source_fit_kernel(SourceFitArgs a) {
     for (int q = threadIdx.x; q < a.nrx; q += blockDim.x) {
         double mk_ = a.mask[t + (int64_t)a.ntx * q] ? 1.0 : 0.0;
         cd p = mkd(0, 0);
-        if (a.rx_x[q] >= 0) p = a.u[((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r];
+        if (a.rx_x[q] >= 0) p = a.u.at(((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r);
         p = mk_ * p;
         cd d = mk_ * a.Y[t + (int64_t)a.ntx * q];
         cd cp = conjd(p) * d;
@@ -366,7 +361,7 @@ __global__ void __launch_bounds__(256) source_fit_kernel(SourceFitArgs a) {
     for (int q = threadIdx.x; q < a.nrx; q += blockDim.x) {
         double mk_ = a.mask[t + (int64_t)a.ntx * q] ? 1.0 : 0.0;
         cd p = mkd(0, 0);
-        if (a.rx_x[q] >= 0) p = a.u[((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r];
+        if (a.rx_x[q] >= 0) p = a.u.at(((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r);
         p = mk_ * p;
         cd d = mk_ * a.Y[t + (int64_t)a.ntx * q];
         cd res = mk_ * (p * scale - d);
@@ -393,6 +388,26 @@ __global__ void __launch_bounds__(256) source_fit_kernel(SourceFitArgs a) {
 // node's accumulator in LDS, so every node is summed by one workgroup in a fixed order (no atomics).
 constexpr int GRAD_COLS = 32;
 
+// a field access split into its loads and their use, for a field known to be a c64 pair (PAIR) or
+// an FP64 field, so that a loaded value occupies the registers of its one form
+template <bool PAIR>
+__device__ __forceinline__ FieldRaw field_load(const FieldView &v, int64_t i) {
+    FieldRaw r{};
+    if constexpr (PAIR) {
+        r.a = v.a[i];
+        if (v.b) r.b = v.b[i];
+    } else {
+        r.u = v.u64[i];
+    }
+    return r;
+}
+template <bool PAIR>
+__device__ __forceinline__ cd field_value(const FieldView &v, const FieldRaw &r) {
+    if constexpr (PAIR) return v.b ? tod(r.a) + tod(r.b) : tod(r.a);
+    else return r.u;
+}
+
+template <bool PAIR_U, bool PAIR_L>
 __global__ void __launch_bounds__(256) gradient_kernel(GradientArgs a) {
     const int lr = threadIdx.x & 31, ly = threadIdx.x >> 5;
     const int y0 = blockIdx.x * 8, yi = y0 + ly;
@@ -414,23 +429,27 @@ __global__ void __launch_bounds__(256) gradient_kernel(GradientArgs a) {
         const bool rin = r < a.R;
         const int rc = min(r, Rpad - 1);              // in-range address for lanes beyond the RHS
         const cd sc = rin ? a.scale[r] : mkd(0, 0);
-        cd pl = mkd(0, 0), hl = mkd(0, 0);            // loaded column: own row, halo row
+        // loaded column (own row, halo row), kept as loaded until it is stored into the ring
+        FieldRaw pl{}, hl{};
+        bool pl_in = false, hl_in = false;
         auto load_col = [&](int x) {
             const bool xin = x >= 0 && x < m;
             const int64_t base = (int64_t)min(max(x, 0), m - 1) * colstride + rc;
-            pl = a.lam[base + (int64_t)own_y * Rpad];
-            if (!(xin && own_in && rin)) pl = mkd(0, 0);
+            pl = field_load<PAIR_L>(a.lam, base + (int64_t)own_y * Rpad);
+            pl_in = xin && own_in && rin;
             if (halo) {
-                hl = a.lam[base + (int64_t)halo_y * Rpad];
-                if (!(xin && halo_in && rin)) hl = mkd(0, 0);
+                hl = field_load<PAIR_L>(a.lam, base + (int64_t)halo_y * Rpad);
+                hl_in = xin && halo_in && rin;
             }
         };
         auto store_col = [&](int x) {
             const int sl = x & 3;
-            ring[sl][ly + 1][lr] = pl;
-            if (halo) ring[sl][hj][lr] = hl;
+            ring[sl][ly + 1][lr] = pl_in ? field_value<PAIR_L>(a.lam, pl) : mkd(0, 0);
+            if (halo) ring[sl][hj][lr] = hl_in ? field_value<PAIR_L>(a.lam, hl) : mkd(0, 0);
         };
-        auto load_u = [&](int x) { return a.u[(int64_t)min(x, m - 1) * colstride + (int64_t)own_y * Rpad + rc]; };
+        auto load_u = [&](int x) {
+            return field_load<PAIR_U>(a.u, (int64_t)min(x, m - 1) * colstride + (int64_t)own_y * Rpad + rc);
+        };
         auto load_v = [&](int x) { return a.vsm[(int64_t)(min(x, m - 1) + 1) * a.ny + (own_y + 1)]; };
         __syncthreads();                              // the previous chunk's last reads of the ring
         load_col(xa - 1);
@@ -438,11 +457,12 @@ __global__ void __launch_bounds__(256) gradient_kernel(GradientArgs a) {
         load_col(xa);
         store_col(xa);
         load_col(xa + 1);
-        cd un = load_u(xa), vn = load_v(xa);
+        FieldRaw un = load_u(xa);
+        cd vn = load_v(xa);
 #pragma unroll 1
         for (int x = xa; x < xb; ++x) {
             store_col(x + 1);
-            const cd u = un, v = vn;
+            const cd u = field_value<PAIR_U>(a.u, un), v = vn;
             load_col(x + 2);
             un = load_u(x + 1);
             vn = load_v(x + 1);
@@ -493,7 +513,7 @@ __global__ void __launch_bounds__(256) curvature_source_kernel(CurvSrcArgs a) {
         if (yy < 0 || yy >= a.n || xx < 0 || xx >= a.m) continue;
         double kw = (dd == DC) ? kc : ((dd == DN || dd == DS || dd == DW || dd == DE) ? ke : kd);
         int64_t node = (int64_t)(xx + 1) * a.ny + (yy + 1);
-        cd v = a.vsm[node] * a.u[((int64_t)xx * a.npad + yy) * a.Rpad + r];
+        cd v = a.vsm[node] * a.u.at(((int64_t)xx * a.npad + yy) * a.Rpad + r);
         s = s + (kw * a.direction[node]) * v;
     }
     a.src[o] = s;
@@ -507,7 +527,7 @@ __global__ void __launch_bounds__(256) curvature_gather_kernel(CurvGatherArgs a)
     double s = 0;
     for (int q = threadIdx.x; q < a.nrx; q += blockDim.x) {
         if (!a.mask[t + (int64_t)a.ntx * q] || a.rx_x[q] < 0) continue;
-        cd u = a.u[((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r];
+        cd u = a.u.at(((int64_t)a.rx_x[q] * a.npad + a.rx_y[q]) * a.Rpad + r);
         s += abs2(u * a.scale[r]);
     }
     s = block_sum(s, sh);
@@ -560,16 +580,10 @@ void ext_to_solve64(const cd *ext, cd *out, int ny, int nx, int npad, int R, int
     hipLaunchKernelGGL(ext_to_solve_kernel<cd>, grid, dim3(256), 0, s, ext, out, ny, nx, n, npad, R, Rpad, norm2);
     WUST_LAUNCH_CHECK();
 }
-void solve64_to_ext(const cd *in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s) {
+void solve_to_ext(const FieldView &in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s) {
     int n = ny - 2;
     dim3 grid((ny + 31) / 32, (R + 31) / 32, nx);
-    hipLaunchKernelGGL(solve_to_ext_kernel<cd>, grid, dim3(256), 0, s, in, ext, ny, nx, n, npad, R, Rpad);
-    WUST_LAUNCH_CHECK();
-}
-void solve32_to_ext(const cf *in, cd *ext, int ny, int nx, int npad, int R, int Rpad, hipStream_t s) {
-    int n = ny - 2;
-    dim3 grid((ny + 31) / 32, (R + 31) / 32, nx);
-    hipLaunchKernelGGL(solve_to_ext_kernel<cf>, grid, dim3(256), 0, s, in, ext, ny, nx, n, npad, R, Rpad);
+    hipLaunchKernelGGL(solve_to_ext_kernel, grid, dim3(256), 0, s, in, ext, ny, nx, n, npad, R, Rpad);
     WUST_LAUNCH_CHECK();
 }
 void launch_residual(const ResidualArgs &a, hipStream_t s) {
@@ -586,10 +600,6 @@ void launch_residual(const ResidualArgs &a, hipStream_t s) {
     else if (a.d0) launch_k(PC_RESIDUAL, residual_kernel<false, true, false>, grid, dim3(256), 0, s, k);
     else if (a.d32) launch_k(PC_RESIDUAL, residual_kernel<false, false, true>, grid, dim3(256), 0, s, k);
     else launch_k(PC_RESIDUAL, residual_kernel<false, false, false>, grid, dim3(256), 0, s, k);
-}
-void promote(const cf *in, cd *out, int64_t count, hipStream_t s) {
-    hipLaunchKernelGGL(promote_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, in, out, count);
-    WUST_LAUNCH_CHECK();
 }
 void demote_flag(const cd *in, cf *out, int64_t count, int64_t per, int *flags, uint8_t *mask, hipStream_t s) {
     require(count % 64 == 0 && per % 64 == 0, "demote_flag: sizes must be multiples of 64");
@@ -641,7 +651,12 @@ void launch_gradient(const GradientArgs &a, hipStream_t s) {
     const int64_t tiles = (int64_t)(a.npad / 8) * a.m;
     k.cols = (int)std::max<int64_t>(1, std::min<int64_t>(GRAD_COLS, tiles / 2048));
     dim3 grid(a.npad / 8, (a.m + k.cols - 1) / k.cols);
-    launch_k(PC_GRADIENT, gradient_kernel, grid, dim3(256), 0, s, k);
+    require((a.u.u64 || a.u.a) && (a.lam.u64 || a.lam.a), "gradient: missing field");
+    const bool pu = !a.u.u64, pl = !a.lam.u64;
+    if (pu && pl) launch_k(PC_GRADIENT, gradient_kernel<true, true>, grid, dim3(256), 0, s, k);
+    else if (pu) launch_k(PC_GRADIENT, gradient_kernel<true, false>, grid, dim3(256), 0, s, k);
+    else if (pl) launch_k(PC_GRADIENT, gradient_kernel<false, true>, grid, dim3(256), 0, s, k);
+    else launch_k(PC_GRADIENT, gradient_kernel<false, false>, grid, dim3(256), 0, s, k);
 }
 void launch_nonfinite_flag(const double *v, int64_t count, int *flag, hipStream_t s) {
     if (count <= 0) return;
