@@ -9,6 +9,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "engine.h"
@@ -165,6 +166,21 @@ void stencil_params(double vmin, double vmax, double f, double h, double g, doub
     d = x0;
     e = x1;
 }
+
+// Whether the last refined solve of an operator shape, frequency and direction needed more than one
+// correction (1088^2 x 512 at 0.3-1.4 MHz: every solve; 560^2 x 256: none).  It decides whether the
+// pass after the first correction stores U and R at once or only measures the residual; both give
+// the same values, so the choice changes the work only.
+struct CorrectionKey {
+    int ny, nx, npad, Rpad, adjoint;
+    double f, dx, dy;
+    bool operator<(const CorrectionKey &o) const {
+        return std::tie(ny, nx, npad, Rpad, adjoint, f, dx, dy) <
+               std::tie(o.ny, o.nx, o.npad, o.Rpad, o.adjoint, o.f, o.dx, o.dy);
+    }
+};
+std::mutex g_corrections_mu;
+std::map<CorrectionKey, bool> g_more_corrections;
 
 struct Retained {
     cd *buf = nullptr;     // storage of the forward field (c128-sized)
@@ -454,20 +470,28 @@ struct wust_helmholtz {
         // precision in out's halves, and the pass after the first correction only measures the residual
         // of D_1 + D_2: with it below tol (the usual case) the result is that pair, and neither the FP64
         // sum nor the residual is stored.  Otherwise the pass runs again to store both (the same
-        // values), and later passes accumulate in FP64, alternating between two buffers ordered so
-        // that the result ends in `out`.
+        // values; it stores them at once where the last solve of this key needed another correction
+        // too), and later passes accumulate in FP64, alternating between two buffers ordered so that
+        // the result ends in `out`.
         double rel = residual_pass(nullptr, nullptr, d0, nullptr, r32);
         st.initial_residual = st.final_residual = rel;
         int it = 0;
+        const CorrectionKey ckey{op.ny, op.nx, op.npad, Rpad, adjoint ? 1 : 0, op.ap.f, op.ap.dx, op.ap.dy};
         if (rel > tol && max_iter > 0) {
             solve_engine(op, r32, d1, b32, Rpad, adjoint, s);
             it = 1;
             res.b = d1;
-            rel = st.final_residual = residual_pass(nullptr, d0, d1, nullptr, nullptr);
+            bool store = false;
+            {
+                std::lock_guard<std::mutex> lk(g_corrections_mu);
+                auto f = g_more_corrections.find(ckey);
+                store = f != g_more_corrections.end() && f->second;
+            }
+            cd *bufs[2] = {t64 == out ? u64 : t64, out};
+            rel = st.final_residual = residual_pass(nullptr, d0, d1, store ? bufs[0] : nullptr, store ? r32 : nullptr);
             if (rel > tol && it < max_iter) {
-                cd *bufs[2] = {t64 == out ? u64 : t64, out};
                 int cur = 0;
-                rel = residual_pass(nullptr, d0, d1, bufs[cur], r32);
+                if (!store) rel = residual_pass(nullptr, d0, d1, bufs[cur], r32);
                 while (rel > tol && it < max_iter) {
                     solve_engine(op, r32, d32, b32, Rpad, adjoint, s);
                     ++it;
@@ -482,6 +506,10 @@ struct wust_helmholtz {
             }
         }
         st.iterations = it;
+        if (it >= 1 && std::isfinite(st.final_residual)) {
+            std::lock_guard<std::mutex> lk(g_corrections_mu);
+            g_more_corrections[ckey] = it >= 2;
+        }
         if (!std::isfinite(st.final_residual))
             throw Error(WUST_NUMERICAL_FAILURE, "non-finite residual in the refined solve");
         if (st.final_residual > tol && max_iter > 0)
