@@ -104,3 +104,30 @@ def test_curvature_after_shrinking_evaluations(case, obs):
     kf = _curv(f, direction, Mf)
     assert _close(l, lf, 1e-10)
     assert kf > 0 and _close(k, kf, 1e-9), (k, kf)
+
+
+@pytest.mark.parametrize("tol, min_iters", [(1e-10, 1), (1e-13, 2)])
+def test_refinement_paths_agree(case, tol, min_iters):
+    """A refined solve's result is the pair (first iterate, correction) when one correction meets the
+    tolerance, and an FP64 field accumulated over further corrections otherwise; after a solve that
+    needed more than one correction the pass that follows the first correction stores U and R at
+    once instead of measuring first.  Repeated solves on one handle (both orders of those paths, the
+    first one also on a fresh key) and a fresh handle give the same solution bit for bit, in both
+    directions, and the solution meets the tolerance in the oracle's sense (stats).  The handles factor
+    with the default Newton-Schulz budgets (``factor='fixed'``), so that both hold the same factor."""
+    rng = np.random.default_rng(11)
+    b = np.zeros((case.ny, case.nx, 6), np.complex128)
+    b[4:-4, 4:-4] = rng.standard_normal((case.ny - 8, case.nx - 8, 6)) + 1j * rng.standard_normal(
+        (case.ny - 8, case.nx - 8, 6))
+    def fixed():
+        return G.HelmholtzSolver(case.x_m, case.y_m, np.full((case.ny, case.nx), 1500.0), None,
+                                 case.frequencies_hz[0], -1, case.cfg["pml_strength"], case.cfg["pml_m"],
+                                 dict(backend="gpu", stencil_bounds=case.cfg["stencil_bounds"], factor="fixed"))
+
+    for adjoint in (False, True):
+        s = fixed()
+        runs = [s.solve(b, adjoint, tol=tol, max_iter=20, return_stats=True) for _ in range(3)]
+        fresh = fixed().solve(b, adjoint, tol=tol, max_iter=20, return_stats=True)
+        for u, _, st in runs + [fresh]:
+            assert st["iterations"] >= min_iters and st["final_residual"] <= tol
+            assert torch.equal(u, runs[0][0])
