@@ -86,3 +86,29 @@ def test_python_mirror_rejects_like_the_reference():
         with pytest.raises(WustError) as ei:
             HelmholtzSolver(x, x, c, np.zeros_like(c), 1e5, -1, 10, .004, bad)
         assert ei.value.identifier == "WUST:InvalidInput"
+
+
+def test_field_shape_guard_on_host():
+    """The Python layer hands the engine plain pointers from which it reads ny * nx * R values, so a
+    field whose leading dimensions are not the grid's is WUST:InvalidInput before any device work
+    (Runtime/solver/applyBlockLU.cu:380-388 rejects such sources); the accepted layout is
+    [ny, nx] or [ny, nx, R], stored as (R, nx, ny)."""
+    torch = pytest.importorskip("torch")
+    from waveforminversionust_amd import helmholtz as G
+    ny, nx = 7, 5
+    good = np.arange(ny * nx * 3, dtype=np.float64).reshape(ny, nx, 3) * (1 + 1j)
+    t = G._field_to_dev(good, "cpu", ny, nx, "bad")
+    assert tuple(t.shape) == (3, nx, ny) and t.dtype == torch.complex128
+    assert torch.equal(t.permute(2, 1, 0), torch.as_tensor(good))
+    assert tuple(G._field_to_dev(good[:, :, 0], "cpu", ny, nx, "bad").shape) == (1, nx, ny)
+    for bad in (np.transpose(good, (2, 0, 1)), good[:-1], good[:, :-1], good[:, :, :0], good[..., None],
+                np.zeros(ny * nx)):
+        with pytest.raises(N.WustError) as ei:
+            G._field_to_dev(bad, "cpu", ny, nx, "Inconsistent Block-LU input shapes.")
+        assert ei.value.identifier == "WUST:InvalidInput"
+    Y = torch.zeros((4, 3), dtype=torch.complex128)    # [n_rx, n_tx] after the transpose
+    M = torch.zeros((4, 3), dtype=torch.uint8)
+    G._check_obs_slice(Y, M, 3, 4)
+    for ntx, nrx in ((4, 3), (3, 5), (2, 4)):
+        with pytest.raises(N.WustError):
+            G._check_obs_slice(Y, M, ntx, nrx)
