@@ -151,7 +151,11 @@ int wust_helmholtz_coefficients(const wust_helmholtz *h, void *Dd, void *Dl, voi
                                 void *Ll, void *Lu, void *Ud, void *Ul, void *Uu, void *stream);
 
 /* Solve H u = src (adjoint = 0) or H^H u = src (adjoint = 1) for R right-hand
- * sides.  d_src, d_out: device complex128 [Ny, Nx, R]; sources must vanish
+ * sides.  d_src, d_out: device complex128 [Ny, Nx, R] with the operator's Ny
+ * and Nx: the library reads and writes exactly Ny * Nx * R values and cannot
+ * check a buffer's size (the reference rejects mismatched shapes,
+ * Runtime/solver/applyBlockLU.cu:380-388; the Python and MEX layers check
+ * them before calling).  Sources must vanish
  * on the outer Dirichlet ring, and the output is zero there (the reference
  * block LU keeps those degrees of freedom at zero,
  * Runtime/solver/applyBlockLU.cu:7-10).  In WUST_MODE_REFINED the FP64
