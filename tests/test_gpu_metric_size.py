@@ -156,7 +156,7 @@ def test_metric_gradient_central_differences(metric):
 
 
 def test_metric_refined_matches_fp64_oracle():
-    """The north-star accuracy gate at the metric size and tolerance: refinement to a 1e-10 relative FP64
+    """The accuracy gate at the metric size and tolerance: refinement to a 1e-10 relative FP64
     residual, gradient evaluated at 1500 m/s with observations from the true model.  On 8 of the 256
     transmitters (all 256 receivers) the refined misfit and slowness gradient match the FP64 sparse-LU
     oracle to <= 1e-8 (Runtime/matlab/wust_oracle.m:28-72); the evaluation of all 256 transmitters in one
